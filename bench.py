@@ -1,0 +1,195 @@
+"""Benchmark of the MRHS even-odd Wilson hopping term on one MI355X.
+
+Workload (the headline line): SU(3), 24^3 x 48 lattice, nrhs = 16, FP32
+storage and arithmetic, packed RHS-innermost fields resident in HBM; a step
+is one application of the hopping term to all 16 right-hand sides, parities
+alternating. value = sites*RHS per second = S * nrhs / t_step.
+
+The JSON line also carries
+  roofline:     algorithmic bytes per launch S*[k*8N*(b_in+b_out) + 16 N^2 b_U]
+                over the kernel's average launch time (HIP events on the
+                launch stream) against 8 TB/s; `traffic` is the rocprofv3
+                FETCH_SIZE + WRITE_SIZE per launch (scaled by a copy-kernel
+                calibration) when a measured summary is committed under profiles/;
+  cpu_baseline: the FP32 C oracle (OpenMP) on the host cores, same workload,
+                a bounded number of applications;
+  extra:        FP16 / BF16 / FP64 storage at the same size and the CG
+                time-to-solution comparison when the solver is built.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_alg(S, k, N, b_in, b_out, b_u):
+    """Compulsory HBM bytes of one application (every field touched once)."""
+    return S * (k * 8 * N * (b_in + b_out) + 16 * N * N * b_u)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
+    ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
+    ap.add_argument("--L", type=int, nargs=4, default=[24, 24, 24, 48])
+    ap.add_argument("--nrhs", type=int, default=16)
+    return ap.parse_args()
+
+
+def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0):
+    """Tune, then time `steps` applications; returns (ms_per_step, tile, bufs)."""
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    from mrhs_qcu_amd import fields
+    import numpy as np
+
+    S = fields.half_volume(L)
+    g = fields.gauge_field(L, N, seed=seed)
+    gu = torch.from_numpy(g.view(np.float64).reshape(-1)).to(dev)
+    del g
+    dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
+    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
+    _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
+    del gu
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
+    b = torch.empty_like(a)
+    best = _lib.Tile()
+    us = ctypes.c_double()
+    _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, ctypes.byref(best),
+                               ctypes.byref(us), sp))
+    bufs = [a, b]
+    for i in range(warmup):
+        _lib.check(lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k,
+                                     i & 1, 0, geom, None, sp))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for i in range(steps):
+        lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k, i & 1, 0,
+                          geom, None, sp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return ms, best.as_tuple()
+
+
+def cpu_baseline(L, N, k, budget_s=12.0):
+    """FP32 C oracle (OpenMP) on the host: same workload, bounded repetitions."""
+    import numpy as np
+
+    from mrhs_qcu_amd import fields
+    from oracle import oracle
+
+    S = fields.half_volume(L)
+    rng = np.random.default_rng(7)
+    g = fields.gauge_field(L, N, seed=0).astype(np.complex64)
+    psi = (rng.standard_normal((k, S, 4, N), dtype=np.float32)
+           + 1j * rng.standard_normal((k, S, 4, N), dtype=np.float32)).astype(np.complex64)
+    out = np.empty_like(psi)
+    oracle.dslash(g, psi, L, 0, 0, single=True, out=out)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        oracle.dslash(g, psi, L, len(times) & 1, 0, single=True, out=out)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s or len(times) >= 50:
+            break
+    t = float(np.median(times))
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return {"value": S * k / t, "unit": "sites*RHS/s", "cores": cores, "kind": "port",
+            "sample": f"FP32 C restatement of the reference applyWilson (oracle/oracle.c), OpenMP, "
+                      f"{'x'.join(map(str, L))} SU({N}) nrhs={k}, median of {len(times)} applications "
+                      f"({sum(times):.1f} s)"}
+
+
+def traffic_from_profile(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from mrhs_qcu_amd import _lib, fields
+
+    lib = _lib.load()
+    L, N, k = tuple(args.L), 3, args.nrhs
+    geom = _lib.geom(L, N)
+    S = fields.half_volume(L)
+
+    ms, tile = time_dslash(lib, _lib, torch, geom, _lib.SINGLE, k, args.steps, args.warmup, L, N)
+    value = S * k / (ms * 1e-3)
+    ba = bytes_alg(S, k, N, 4, 4, 4)
+    achieved = ba / (ms * 1e-3) / 1e9
+    wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_fp32"
+    line = {
+        "metric": "MRHS dslash sites*RHS/s (SU(3) 24^3x48 nrhs=16 FP32) + %HBM roofline",
+        "value": value,
+        "unit": "sites*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SU(3) links, Gaussian spinors)",
+        "config": {"workload": wl, "lattice": list(L), "nColors": N, "nrhs": k, "storage": "fp32",
+                   "tile": {"rhs_per_lane": tile[0], "block": tile[1], "link_lds": tile[2], "t_block": tile[3]},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dslash_c3_fp32_traffic.json"),
+                     "bytes_alg_per_launch": ba},
+    }
+    if not args.profile and not args.no_extra:
+        extra = {}
+        for name, prec, b in (("fp16", _lib.HALF, 2), ("bf16", _lib.BF16, 2), ("fp64", _lib.DOUBLE, 8)):
+            ms2, t2 = time_dslash(lib, _lib, torch, geom, prec, k, max(20, args.steps // 4), 10, L, N)
+            bb = bytes_alg(S, k, N, b, b, b)
+            extra[name] = {"ms_per_step": ms2, "sites_rhs_per_s": S * k / (ms2 * 1e-3),
+                           "hbm_frac": bb / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tile": t2}
+        Lc2, kc2 = (16, 16, 16, 32), 12
+        g2 = _lib.geom(Lc2, N)
+        ms3, t3 = time_dslash(lib, _lib, torch, g2, _lib.SINGLE, kc2, max(50, args.steps // 2), 10, Lc2, N)
+        S2 = fields.half_volume(Lc2)
+        extra["c2_16x16x16x32_nrhs12_fp32"] = {
+            "ms_per_step": ms3, "sites_rhs_per_s": S2 * kc2 / (ms3 * 1e-3),
+            "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "tile": t3}
+        line["extra"] = extra
+    if not args.no_cpu and not args.profile:
+        line["cpu_baseline"] = cpu_baseline(L, N, k)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

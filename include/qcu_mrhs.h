@@ -1,0 +1,140 @@
+/* Extended C ABI of the MI355X MRHS Wilson dslash library.
+ *
+ * The reference exposes only the queue-style entry points of qcu.h; these
+ * entry points expose the same operator on explicit, persistent device
+ * buffers so that a caller (an FFI binding, a solver, a benchmark) can keep
+ * fields resident in HBM in the native layout and avoid any per-call copy.
+ *
+ * Conventions
+ *  - Every field pointer is a HIP device pointer; `stream` is a hipStream_t
+ *    (NULL = the default stream). Calls are asynchronous on `stream` unless
+ *    stated otherwise.
+ *  - Return value 0 = success; nonzero = error, message via qcuMrhsLastError().
+ *  - Precision codes: QCU_MRHS_HALF (IEEE binary16 storage, FP32 arithmetic),
+ *    QCU_MRHS_SINGLE, QCU_MRHS_DOUBLE, QCU_MRHS_BF16 (bfloat16 storage, FP32
+ *    arithmetic). They extend QCU_PRECISION of qcu_enum.h.
+ *  - Layouts (complex numbers interleaved re, im; S = Lx*Ly*Lz*Lt/2 sites of
+ *    one parity, checkerboard index idx = ((t*Lz+z)*Ly+y)*(Lx/2)+x/2):
+ *      user spinor   [S][4][N]            (one buffer per RHS; the qcu.h layout)
+ *      user gauge    [4][2][S][N][N]      (the qcu.h layout)
+ *      packed spinor [S][4][N][nrhs]      (native; RHS innermost)
+ *      native gauge  [2][S][8][N][N]      (per site: U_mu(x) for mu = 0..3, then
+ *                                          U_mu(x - mu)^dagger for mu = 0..3)
+ *  - The operator is the bare hopping term of the reference
+ *    (wilson_dslash_4d.cuh:59-293):
+ *      out(x) = sum_mu U_mu(x)(1 - s g_mu) in(x+mu) + U_mu(x-mu)^+ (1 + s g_mu) in(x-mu)
+ *    for x of the output parity, s = +1 (dagger 0) or -1 (dagger 1).
+ */
+#pragma once
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QCU_MRHS_HALF = 0, QCU_MRHS_SINGLE = 1, QCU_MRHS_DOUBLE = 2, QCU_MRHS_BF16 = 4 };
+
+typedef struct qcuMrhsGeom_s {
+  int L[4];    /* local lattice Lx, Ly, Lz, Lt (all even) */
+  int grid[4]; /* process grid; a dimension with grid > 1 skips the hops that leave the local volume */
+  int nColors; /* N: 2..8 */
+} qcuMrhsGeom;
+
+typedef struct qcuMrhsTile_s {
+  int rhs_per_lane; /* 1, 2 or 4 right-hand sides per lane */
+  int block;        /* lanes per workgroup: 64, 128, 192 or 256 */
+  int link_lds;     /* 1: stage each workgroup's links in LDS */
+  int t_block;      /* sites traversed in blocks of t_block time slices (1 = natural order) */
+} qcuMrhsTile;
+
+const char *qcuMrhsLastError(void);
+int qcuMrhsVersion(void);
+
+size_t qcuMrhsSpinorBytes(const qcuMrhsGeom *g, int nrhs, int prec);
+size_t qcuMrhsGaugeBytes(const qcuMrhsGeom *g, int prec);
+
+/* ---- storage layer ---------------------------------------------------- */
+/* user gauge (FP64 or FP32) -> native gauge in dst_prec. */
+int qcuMrhsPackGauge(void *dst, int dst_prec, const void *src_user, int src_prec, const qcuMrhsGeom *g, void *stream);
+/* nrhs user spinors (host array of device pointers) -> packed spinor. */
+int qcuMrhsPackSpinors(void *dst_packed, int dst_prec, const void *const *src_user, int src_prec, int nrhs,
+                       const qcuMrhsGeom *g, void *stream);
+/* packed spinor -> nrhs user spinors. */
+int qcuMrhsUnpackSpinors(void *const *dst_user, int dst_prec, const void *src_packed, int src_prec, int nrhs,
+                         const qcuMrhsGeom *g, void *stream);
+/* n complex numbers converted element-wise between precisions. */
+int qcuMrhsConvert(void *dst, int dst_prec, const void *src, int src_prec, size_t n, void *stream);
+/* Reference shift semantics for any N and element width (bytes per complex:
+ * 4, 8 or 16): direction 0 = user -> reference SoA, 1 = SoA -> user. */
+int qcuMrhsShiftSpinor(void *dst, const void *src, int direction, const qcuMrhsGeom *g, int bytes_per_complex,
+                       void *stream);
+int qcuMrhsShiftGauge(void *dst, const void *src, int direction, const qcuMrhsGeom *g, int bytes_per_complex,
+                      void *stream);
+
+/* ---- operator ----------------------------------------------------------- */
+/* out = D in on packed fields; gauge in the native layout; all three in prec.
+ * tile = NULL: the tuned (or default) tile for this shape. */
+int qcuMrhsDslash(void *out, const void *in, const void *gauge, int prec, int nrhs, int parity, int dagger,
+                  const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
+/* out = a * x + b * D in (x packed, same precision and parity as out). */
+int qcuMrhsDslashXpay(void *out, const void *in, const void *x, double a, double b, const void *gauge, int prec,
+                      int nrhs, int parity, int dagger, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
+/* Hopping term on nrhs user spinors (host arrays of device pointers), native
+ * gauge in prec; the qcu.h startDslash path without the queue. */
+int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const void *gauge, int prec, int nrhs,
+                      int parity, int dagger, const qcuMrhsGeom *g, void *stream);
+/* Hopping term through qcu::instantiateDslash: spinors in the reference SoA
+ * layout (host arrays of device pointers), gauge in the reference SoA layout;
+ * prec is QCU_MRHS_HALF, _SINGLE or _DOUBLE. Synchronous. */
+int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void *gauge_soa, int prec, int nrhs,
+                                 int parity, int dagger, const qcuMrhsGeom *g);
+
+/* ---- tuning ------------------------------------------------------------- */
+/* Times every instantiated tile on the given fields (out is overwritten),
+ * records the fastest for (N, nrhs, prec, lattice) and returns it. */
+int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *g,
+                qcuMrhsTile *best, double *best_us, void *stream);
+/* Tile the dispatcher uses for a shape (tuned if known, else the default). */
+int qcuMrhsGetTile(const qcuMrhsGeom *g, int prec, int nrhs, qcuMrhsTile *tile);
+/* Number of instantiated dslash kernels (for diagnostics). */
+int qcuMrhsKernelCount(void);
+
+/* ---- solver ------------------------------------------------------------- */
+typedef struct qcuMrhsCgParam_s {
+  double kappa;          /* hopping parameter; M^ = 1 - kappa^2 D_eo D_oe */
+  double tol;            /* target ||M^+b - M^+M^ x|| / ||M^+b|| per RHS */
+  int max_iter;          /* iteration cap */
+  int reliable_interval; /* mixed precision: iterations between FP64 residual updates */
+  int sloppy_prec;       /* QCU_MRHS_DOUBLE: FP64 CG; HALF / BF16 / SINGLE: mixed precision */
+  /* outputs */
+  int iterations;
+  int reliable_updates;
+  double seconds;
+} qcuMrhsCgParam;
+
+/* Solves M^+ M^ x = M^+ b on the even checkerboard for nrhs right-hand sides.
+ * x, b: packed FP64 even-parity spinors; gauge: native FP64 gauge.
+ * true_res (host, nrhs doubles, may be NULL) receives the final true relative
+ * residual of each RHS recomputed in FP64. Synchronous. */
+int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, const qcuMrhsGeom *g, qcuMrhsCgParam *p,
+                   double *true_res, void *stream);
+
+/* ---- batching ------------------------------------------------------------ */
+typedef struct qcuMrhsProblem_s {
+  qcuMrhsGeom geom;
+  int nrhs, parity, dagger;
+  void *out;         /* packed FP32 */
+  const void *in;    /* packed FP32 */
+  const void *gauge; /* native FP32 */
+} qcuMrhsProblem;
+
+/* Builds one replayable launch (a HIP graph) applying the dslash to every
+ * problem, with a tuned tile per problem. */
+int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, void *stream);
+int qcuMrhsBatchRun(void *handle, void *stream);
+int qcuMrhsBatchDestroy(void *handle);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,214 @@
+// Validation and launch of the dslash kernel family.
+#include "dslash_dispatch.h"
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+namespace qmrhs {
+
+namespace {
+struct Entry {
+  KernelKey key;
+  LaunchFn fn;
+  AttrFn attr;
+};
+std::vector<Entry> &registry()
+{
+  static std::vector<Entry> r;
+  return r;
+}
+std::mutex &registry_mutex()
+{
+  static std::mutex m;
+  return m;
+}
+
+int bytes_of(int prec)
+{
+  switch (prec) {
+    case PREC_DOUBLE: return 16;
+    case PREC_SINGLE: return 8;
+    case PREC_HALF: return 4;
+    case PREC_BF16: return 4;
+    default: return 0;
+  }
+}
+}  // namespace
+
+void register_kernel(const KernelKey &key, LaunchFn fn, AttrFn attr)
+{
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  registry().push_back(Entry{key, fn, attr});
+}
+
+LaunchFn find_kernel(const KernelKey &key)
+{
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  for (const Entry &e : registry())
+    if (e.key == key) return e.fn;
+  return nullptr;
+}
+
+AttrFn find_kernel_attr(const KernelKey &key)
+{
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  for (const Entry &e : registry())
+    if (e.key == key) return e.attr;
+  return nullptr;
+}
+
+int registry_size()
+{
+  std::lock_guard<std::mutex> lk(registry_mutex());
+  return int(registry().size());
+}
+
+Tile default_tile(int N, int k, int prec)
+{
+  Tile t;
+  t.block = 256;
+  t.tblock = 1;
+  t.R = 1;
+  t.lds = 0;
+  if (k % 2 == 0 && N <= 5 && prec != PREC_DOUBLE) t.R = 2;
+  if (k / t.R >= 4 && N <= 5) t.lds = 1;
+  return t;
+}
+
+static bool fail(std::string *err, const std::string &m)
+{
+  if (err) *err = m;
+  return false;
+}
+
+static bool check_geom(const Geom &g, std::string *err)
+{
+  for (int d = 0; d < 4; ++d) {
+    if (g.L[d] <= 0 || (g.L[d] & 1)) return fail(err, "lattice extents must be positive and even");
+    if (g.grid[d] <= 0) return fail(err, "process grid extents must be positive");
+  }
+  if (g.N < 2 || g.N > 8) return fail(err, "nColors must be in 2..8");
+  const long long V = 1LL * g.L[0] * g.L[1] * g.L[2] * g.L[3];
+  if (V / 2 >= (1LL << 30)) return fail(err, "local volume too large for 32-bit site indexing");
+  return true;
+}
+
+static void fill_common(DslashArgs &a, const DslashCall &c)
+{
+  a.out = c.out;
+  a.in = c.in;
+  a.gauge = c.gauge;
+  a.x = c.x;
+  a.sc_in = c.sc_in;
+  a.sc_x = c.sc_x;
+  a.sc_out = c.sc_out;
+  a.xa = c.xa;
+  a.xb = c.xb;
+  a.partial = c.partial;
+  a.sign = c.dagger ? -1.0f : 1.0f;
+  a.parity = c.parity;
+  a.subLx = c.geom.L[0] / 2;
+  a.Ly = c.geom.L[1];
+  a.Lz = c.geom.L[2];
+  a.Lt = c.geom.L[3];
+  a.S = a.subLx * a.Ly * a.Lz * a.Lt;
+  for (int d = 0; d < 4; ++d) a.skip[d] = c.geom.grid[d] > 1 ? 1 : 0;
+  a.tblock = c.tile.tblock < 1 ? 1 : c.tile.tblock;
+  a.div_subLx = make_fastdiv(uint32_t(a.subLx));
+  a.div_row = make_fastdiv(uint32_t(a.subLx * a.Ly));
+  a.div_Lz = make_fastdiv(uint32_t(a.Lz));
+  a.div_tb = make_fastdiv(uint32_t(a.tblock));
+  a.div_S = make_fastdiv(uint32_t(a.S));
+}
+
+long dslash_num_blocks(const DslashCall &c)
+{
+  const long S = long(c.geom.L[0]) * c.geom.L[1] * c.geom.L[2] * c.geom.L[3] / 2;
+  const int R = c.sl == SL_PACKED ? c.tile.R : 1;
+  const long lanes = S * (c.k / (R > 0 ? R : 1));
+  const int B = c.tile.block > 0 ? c.tile.block : 256;
+  return (lanes + B - 1) / B;
+}
+
+int launch_dslash(const DslashCall &c, std::string *err)
+{
+  if (!check_geom(c.geom, err)) return 1;
+  if (c.k < 1) return fail(err, "nrhs must be >= 1"), 1;
+  if (c.parity != 0 && c.parity != 1) return fail(err, "parity must be 0 or 1"), 1;
+  if (!bytes_of(c.prec_s) || !bytes_of(c.prec_g) || !bytes_of(c.prec_o))
+    return fail(err, "unsupported storage precision"), 1;
+  const int B = c.tile.block;
+  if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
+  if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "tblock must divide Lt"), 1;
+  if (!c.gauge) return fail(err, "null gauge"), 1;
+
+  DslashArgs a{};
+  fill_common(a, c);
+
+  if (c.sl == SL_PACKED) {
+    const int R = c.tile.R;
+    if (R != 1 && R != 2 && R != 4) return fail(err, "rhs_per_lane must be 1, 2 or 4"), 1;
+    if (c.k % R) return fail(err, "nrhs must be a multiple of rhs_per_lane"), 1;
+    if (!c.out || !c.in) return fail(err, "null spinor"), 1;
+    const uintptr_t al = uintptr_t(16);
+    if ((uintptr_t(c.out) | uintptr_t(c.in) | uintptr_t(c.gauge) | uintptr_t(c.x)) % al)
+      return fail(err, "packed fields must be 16-byte aligned"), 1;
+    if (c.partial && c.k > 256) return fail(err, "norm partials support at most 256 RHS"), 1;
+    if (c.gl != GL_NATIVE) return fail(err, "packed spinors use the native gauge layout"), 1;
+    const int lds = c.tile.lds ? 1 : 0;
+    KernelKey key{SL_PACKED, GL_NATIVE, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
+    LaunchFn fn = find_kernel(key);
+    if (!fn) return fail(err, "no kernel instantiated for this precision / N / tile"), 1;
+    a.k = c.k;
+    a.P = c.k / R;
+    a.div_P = make_fastdiv(uint32_t(a.P));
+    const unsigned long long lanes = 1ULL * a.S * a.P;
+    if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
+    a.nlanes = uint32_t(lanes);
+    a.nblocks = uint32_t((lanes + B - 1) / B);
+    size_t shm = 0;
+    if (lds) {
+      const size_t nsite_max = size_t((B - 1) / a.P + 2);
+      shm = nsite_max * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
+      if (shm > 65536) return fail(err, "LDS link staging exceeds 64 KiB for this tile"), 1;
+    }
+    hipError_t e = fn(a, dim3(a.nblocks), dim3(B), shm, c.stream);
+    if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
+    return 0;
+  }
+
+  // Pointer layouts: one RHS per lane, at most kMaxPtrs RHS per launch.
+  if (!c.out_ptrs || !c.in_ptrs) return fail(err, "null pointer arrays"), 1;
+  if (c.x || c.sc_out || c.sc_x || c.partial) return fail(err, "xpay / scaling / norms need the packed layout"), 1;
+  const bool sm = c.sl == SL_SOA_PTRS;
+  if (c.sl == SL_AOS_PTRS && c.gl != GL_NATIVE) return fail(err, "AoS spinors use the native gauge layout"), 1;
+  if (c.sl == SL_SOA_PTRS && c.gl != GL_REF_SOA) return fail(err, "SoA spinors use the reference SoA gauge"), 1;
+  if (c.sl != SL_AOS_PTRS && c.sl != SL_SOA_PTRS) return fail(err, "unknown spinor layout"), 1;
+  KernelKey key{c.sl, c.gl, c.geom.N, 1, 0, sm ? 1 : 0, c.prec_s, c.prec_g, c.prec_o};
+  LaunchFn fn = find_kernel(key);
+  if (!fn) return fail(err, "no kernel instantiated for this layout / precision / N"), 1;
+  for (int r0 = 0; r0 < c.k; r0 += kMaxPtrs) {
+    const int kk = (c.k - r0) < kMaxPtrs ? (c.k - r0) : kMaxPtrs;
+    DslashArgs b = a;
+    for (int r = 0; r < kk; ++r) {
+      if (!c.out_ptrs[r0 + r] || !c.in_ptrs[r0 + r]) return fail(err, "null spinor pointer"), 1;
+      b.out_ptrs.p[r] = c.out_ptrs[r0 + r];
+      b.in_ptrs.p[r] = const_cast<void *>(c.in_ptrs[r0 + r]);
+    }
+    b.k = kk;
+    b.P = kk;
+    b.div_P = make_fastdiv(uint32_t(kk));
+    b.tblock = 1;
+    b.div_tb = make_fastdiv(1);
+    const unsigned long long lanes = 1ULL * b.S * kk;
+    if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
+    b.nlanes = uint32_t(lanes);
+    b.nblocks = uint32_t((lanes + B - 1) / B);
+    hipError_t e = fn(b, dim3(b.nblocks), dim3(B), 0, c.stream);
+    if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
+  }
+  return 0;
+}
+
+}  // namespace qmrhs

@@ -1,0 +1,91 @@
+// Host-side launch interface of the dslash kernel family: a registry maps a
+// kernel key (layouts, precisions, N, tile) to the instantiated launcher, and
+// launch_dslash() validates a call, fills DslashArgs and launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "dslash_kernel.h"
+
+namespace qmrhs {
+
+struct KernelKey {
+  int sl, gl, n, r, lds, site_minor, prec_s, prec_g, prec_o;
+  bool operator==(const KernelKey &o) const
+  {
+    return sl == o.sl && gl == o.gl && n == o.n && r == o.r && lds == o.lds && site_minor == o.site_minor &&
+           prec_s == o.prec_s && prec_g == o.prec_g && prec_o == o.prec_o;
+  }
+};
+
+using LaunchFn = hipError_t (*)(const DslashArgs &, dim3, dim3, size_t, hipStream_t);
+using AttrFn = hipError_t (*)(hipFuncAttributes *);
+
+void register_kernel(const KernelKey &key, LaunchFn fn, AttrFn attr);
+LaunchFn find_kernel(const KernelKey &key);
+AttrFn find_kernel_attr(const KernelKey &key);
+int registry_size();
+
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+hipError_t launch_impl(const DslashArgs &a, dim3 g, dim3 b, size_t shm, hipStream_t st)
+{
+  hipLaunchKernelGGL((dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A>), g, b, shm, st, a);
+  return hipGetLastError();
+}
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+hipError_t attr_impl(hipFuncAttributes *at)
+{
+  return hipFuncGetAttributes(at, reinterpret_cast<const void *>(&dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A>));
+}
+
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+struct Reg {
+  Reg()
+  {
+    register_kernel(KernelKey{SL, GL, N, R, LDS ? 1 : 0, SM ? 1 : 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec,
+                              StoreInfo<CO>::prec},
+                    &launch_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A>, &attr_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A>);
+  }
+};
+
+// Geometry of one lattice (local volume + process grid) and colour count.
+struct Geom {
+  int L[4];
+  int grid[4];
+  int N;
+};
+
+// Tile shape of a launch.
+struct Tile {
+  int R = 1;       // RHS per lane
+  int block = 256; // lanes per workgroup
+  int lds = 0;     // stage links in LDS
+  int tblock = 1;  // traversal in blocks of tblock time slices
+};
+
+struct DslashCall {
+  int sl = SL_PACKED, gl = GL_NATIVE;
+  int prec_s = PREC_SINGLE, prec_g = PREC_SINGLE, prec_o = PREC_SINGLE;
+  Geom geom{};
+  Tile tile{};
+  int k = 1, parity = 0, dagger = 0;
+  void *out = nullptr;
+  const void *in = nullptr, *gauge = nullptr, *x = nullptr;
+  void *const *out_ptrs = nullptr;       // host arrays of k device pointers (pointer layouts)
+  const void *const *in_ptrs = nullptr;
+  const float *sc_in = nullptr, *sc_x = nullptr, *sc_out = nullptr;
+  double xa = 0.0, xb = 1.0;
+  double *partial = nullptr;  // [nblocks][k]; nblocks from dslash_num_blocks()
+  hipStream_t stream = nullptr;
+};
+
+// Returns 0 on success; otherwise a message in *err (when non-null).
+int launch_dslash(const DslashCall &c, std::string *err);
+// Workgroups a packed-layout launch uses (sizing of the partial-sum buffer).
+long dslash_num_blocks(const DslashCall &c);
+// Default tile for a packed launch (used when the autotuner has no entry).
+Tile default_tile(int N, int k, int prec);
+
+}  // namespace qmrhs

@@ -1,0 +1,423 @@
+// Multi-right-hand-side even-odd Wilson hopping term for SU(N) on gfx950.
+//
+// Operator (reference include/kernels/wilson_dslash_4d.cuh:59-293): for an
+// output site x of parity p and s = +1 (daggerFlag 0) / -1 (daggerFlag 1),
+//   out(x) = sum_mu [ U_mu(x) (1 - s g_mu) in(x+mu) + U_mu(x-mu)^+ (1 + s g_mu) in(x-mu) ]
+// in the DeGrand-Rossi basis, periodic in all four directions, with the terms
+// that cross the local boundary skipped in every dimension whose process grid
+// is larger than one (reference wilson_dslash_4d.cuh:60,87,118,147,177,206,237,266).
+//
+// Each hop is done on half spinors: the 4N input components are projected to
+// two colour vectors h0 = in_0 + k0 s in_a0 and h1 = in_1 + k1 s in_a1, both are
+// multiplied by the link V (U_mu(x) forward, U_mu(x-mu)^+ backward), and the
+// result is reconstructed as out_0 += V h0, out_1 += V h1,
+// out_a0 += conj(k0) s V h0, out_a1 += conj(k1) s V h1 (table kHop below).
+//
+// Work decomposition: one lane owns one output site and R consecutive
+// right-hand sides, so the link of a site is fetched once per R RHS (and once
+// per workgroup when it is staged in LDS), and in the native layout the 8
+// neighbour spinor blocks are read as contiguous RHS-innermost vectors.
+#pragma once
+
+#include "qcu_device.h"
+
+namespace qmrhs {
+
+enum SpinorLayout : int {
+  SL_PACKED = 0,    // native: [S][4][N][k] complex, RHS innermost
+  SL_AOS_PTRS = 1,  // one pointer per RHS, each the user layout [S][4][N]
+  SL_SOA_PTRS = 2,  // one pointer per RHS, each the reference SoA layout [4*N][S]
+};
+enum GaugeLayout : int {
+  GL_NATIVE = 0,   // [2][S][8][N][N]: per site 4 forward links then 4 daggered backward links
+  GL_REF_SOA = 1,  // reference SoA [4][2][N*N][S]; backward links daggered on the fly
+};
+
+constexpr int kMaxPtrs = 32;  // RHS per launch for the pointer layouts
+struct PtrPack {
+  void *p[kMaxPtrs];
+};
+
+struct DslashArgs {
+  void *out;
+  const void *in;
+  const void *gauge;
+  const void *x;          // xpay operand (same layout and type as in), or null
+  PtrPack out_ptrs, in_ptrs;
+  const float *sc_in;     // per-RHS: true input = stored input * sc_in[r]      (null = 1)
+  const float *sc_x;      // per-RHS: true x     = stored x     * sc_x[r]       (null = 1)
+  const float *sc_out;    // per-RHS: stored out = true out / sc_out[r]         (null = 1)
+  double xa, xb;          // true out = xa * x + xb * D(in)   (x null: xb * D(in))
+  double *partial;        // per-workgroup per-RHS sum of |true out|^2, [nblocks][k] (nullable)
+  float sign;             // +1 (daggerFlag 0) or -1 (daggerFlag 1)
+  int k;                  // RHS count (RHS stride of the packed layout)
+  int P;                  // lanes per site = k / R
+  int parity;             // output parity
+  int subLx, Ly, Lz, Lt;
+  int S;                  // sites per parity
+  int skip[4];            // dimension has grid > 1
+  int tblock;             // traversal: blocks of tblock time slices (1 = natural order)
+  uint32_t nlanes;        // S * P
+  uint32_t nblocks;
+  FastDiv div_P, div_subLx, div_row, div_Lz, div_tb, div_S;
+};
+
+// Hop table: direction D = 2*mu + (backward); projection partner spins a0/a1
+// and phases k0/k1 (0: +1, 1: -1, 2: +i, 3: -i). Read off the reference's
+// projector lines wilson_dslash_4d.cuh:74-81 (X+), 102-112 (X-), 134-141 (Y+),
+// 162-171 (Y-), 193-200 (Z+), 222-231 (Z-), 253-260 (T+), 281-290 (T-).
+struct HopEntry {
+  int a0, k0, a1, k1;
+};
+__host__ __device__ constexpr HopEntry hop_entry(int D)
+{
+  switch (D) {
+    case 0: return {3, 3, 2, 3};  // X+ : h0 = in0 - i s in3, h1 = in1 - i s in2
+    case 1: return {3, 2, 2, 2};  // X- : h0 = in0 + i s in3, h1 = in1 + i s in2
+    case 2: return {3, 0, 2, 1};  // Y+ : h0 = in0 +   s in3, h1 = in1 -   s in2
+    case 3: return {3, 1, 2, 0};  // Y- : h0 = in0 -   s in3, h1 = in1 +   s in2
+    case 4: return {2, 3, 3, 2};  // Z+ : h0 = in0 - i s in2, h1 = in1 + i s in3
+    case 5: return {2, 2, 3, 3};  // Z- : h0 = in0 + i s in2, h1 = in1 - i s in3
+    case 6: return {2, 1, 3, 1};  // T+ : h0 = in0 -   s in2, h1 = in1 -   s in3
+    default: return {2, 0, 3, 0};  // T- : h0 = in0 +   s in2, h1 = in1 +   s in3
+  }
+}
+
+// Slot of direction D inside a site's 8-link block of the native gauge layout.
+__host__ __device__ constexpr int native_link_slot(int D) { return (D & 1) ? 4 + (D >> 1) : (D >> 1); }
+
+// z * (phase * s)
+template <int KP, class A>
+__device__ __forceinline__ cpx<A> phase_mul(cpx<A> z, A s)
+{
+  if constexpr (KP == 0) return cpx<A>{s * z.re, s * z.im};
+  else if constexpr (KP == 1) return cpx<A>{-s * z.re, -s * z.im};
+  else if constexpr (KP == 2) return cpx<A>{-s * z.im, s * z.re};
+  else return cpx<A>{s * z.im, -s * z.re};
+}
+template <int KP> struct ConjPhase { static constexpr int v = KP < 2 ? KP : (KP == 2 ? 3 : 2); };
+
+template <class A>
+__device__ __forceinline__ void cadd(cpx<A> &a, cpx<A> b)
+{
+  a.re += b.re;
+  a.im += b.im;
+}
+
+// acc += the hop in direction D for one RHS, given the link V and the full
+// neighbour spinor psi[4][N].
+template <int N, int D, class A>
+__device__ __forceinline__ void hop_one(cpx<A> (&acc)[4][N], const cpx<A> (&V)[N][N], const cpx<A> (&psi)[4][N], A s)
+{
+  constexpr HopEntry e = hop_entry(D);
+  cpx<A> h0[N], h1[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    cpx<A> t0 = phase_mul<e.k0>(psi[e.a0][c], s);
+    cpx<A> t1 = phase_mul<e.k1>(psi[e.a1][c], s);
+    h0[c] = cpx<A>{psi[0][c].re + t0.re, psi[0][c].im + t0.im};
+    h1[c] = cpx<A>{psi[1][c].re + t1.re, psi[1][c].im + t1.im};
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    cpx<A> g0 = cz<A>(), g1 = cz<A>();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const cpx<A> v = V[i][j];
+      g0.re += v.re * h0[j].re - v.im * h0[j].im;
+      g0.im += v.re * h0[j].im + v.im * h0[j].re;
+      g1.re += v.re * h1[j].re - v.im * h1[j].im;
+      g1.im += v.re * h1[j].im + v.im * h1[j].re;
+    }
+    cadd(acc[0][i], g0);
+    cadd(acc[1][i], g1);
+    cadd(acc[e.a0][i], phase_mul<ConjPhase<e.k0>::v>(g0, s));
+    cadd(acc[e.a1][i], phase_mul<ConjPhase<e.k1>::v>(g1, s));
+  }
+}
+
+// Site coordinates and neighbours of one output site.
+struct SiteGeo {
+  int idx;       // checkerboard index ((t*Lz+z)*Ly+y)*subLx+xh
+  int nbr[8];    // neighbour index per direction D
+  int live[8];   // 0 if the term is skipped at a local boundary
+};
+
+__device__ __forceinline__ SiteGeo site_geometry(uint32_t sl, const DslashArgs &a, bool natural)
+{
+  SiteGeo g;
+  const uint32_t row = uint32_t(a.subLx) * a.Ly;
+  uint32_t q = fdiv(sl, a.div_row);
+  uint32_t r = sl - q * row;
+  uint32_t y = fdiv(r, a.div_subLx);
+  uint32_t xh = r - y * a.subLx;
+  uint32_t z, t;
+  if (natural) {
+    t = fdiv(q, a.div_Lz);
+    z = q - t * a.Lz;
+  } else {
+    uint32_t q2 = fdiv(q, a.div_tb);
+    uint32_t tin = q - q2 * a.tblock;
+    uint32_t tout = fdiv(q2, a.div_Lz);
+    z = q2 - tout * a.Lz;
+    t = tout * a.tblock + tin;
+  }
+  const int idx = int(((t * a.Lz + z) * a.Ly + y) * a.subLx + xh);
+  g.idx = idx;
+  const int eo = int((y + z + t) & 1u);
+  const int p = a.parity;
+  // x: the checkerboard x index moves only when the full x coordinate crosses a pair.
+  const int xf = (eo != p) ? (int(xh) + 1 == a.subLx ? 0 : int(xh) + 1) : int(xh);
+  const int xb = (eo == p) ? (xh == 0 ? a.subLx - 1 : int(xh) - 1) : int(xh);
+  g.nbr[0] = idx - int(xh) + xf;
+  g.nbr[1] = idx - int(xh) + xb;
+  const int sy = a.subLx, sz = int(row), st = int(row) * a.Lz;
+  g.nbr[2] = idx + (int(y) + 1 == a.Ly ? -(a.Ly - 1) * sy : sy);
+  g.nbr[3] = idx + (y == 0 ? (a.Ly - 1) * sy : -sy);
+  g.nbr[4] = idx + (int(z) + 1 == a.Lz ? -(a.Lz - 1) * sz : sz);
+  g.nbr[5] = idx + (z == 0 ? (a.Lz - 1) * sz : -sz);
+  g.nbr[6] = idx + (int(t) + 1 == a.Lt ? -(a.Lt - 1) * st : st);
+  g.nbr[7] = idx + (t == 0 ? (a.Lt - 1) * st : -st);
+  g.live[0] = !(a.skip[0] && int(xh) == a.subLx - 1 && p != eo);
+  g.live[1] = !(a.skip[0] && xh == 0 && p == eo);
+  g.live[2] = !(a.skip[1] && int(y) == a.Ly - 1);
+  g.live[3] = !(a.skip[1] && y == 0);
+  g.live[4] = !(a.skip[2] && int(z) == a.Lz - 1);
+  g.live[5] = !(a.skip[2] && z == 0);
+  g.live[6] = !(a.skip[3] && int(t) == a.Lt - 1);
+  g.live[7] = !(a.skip[3] && t == 0);
+  return g;
+}
+
+// Load a neighbour spinor (all 4N components) for R right-hand sides.
+template <int N, int R, int SL, class CS, class A>
+__device__ __forceinline__ void load_spinor(cpx<A> (&psi)[R][4][N], const DslashArgs &a, const void *base, int site,
+                                            int slot)
+{
+  if constexpr (SL == SL_PACKED) {
+    const CVec<CS, R> *p = reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(base) +
+                                                                  size_t(site) * 4 * N * a.k + size_t(slot) * R);
+    const size_t stride = size_t(a.k) / R;  // in CVec units between consecutive (spin,colour)
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) {
+      CVec<CS, R> v = p[sc * stride];
+#pragma unroll
+      for (int r = 0; r < R; ++r) psi[r][sc / N][sc % N] = ld_cvt<A>(v.v[r]);
+    }
+  } else if constexpr (SL == SL_AOS_PTRS) {
+    static_assert(R == 1, "pointer layouts hold one RHS per lane");
+    const CS *p = static_cast<const CS *>(a.in_ptrs.p[slot]) + size_t(site) * 4 * N;
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) psi[0][sc / N][sc % N] = ld_cvt<A>(p[sc]);
+  } else {
+    static_assert(R == 1, "pointer layouts hold one RHS per lane");
+    const CS *p = static_cast<const CS *>(a.in_ptrs.p[slot]) + site;
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) psi[0][sc / N][sc % N] = ld_cvt<A>(p[size_t(sc) * a.S]);
+  }
+}
+
+// Load the link of direction D for the output site (native layout: from a
+// site-contiguous block of 8 links, in global memory or staged in LDS).
+template <int N, class CG, class A>
+__device__ __forceinline__ void load_link_native(cpx<A> (&V)[N][N], const CG *blk8, int D)
+{
+  const CG *p = blk8 + D * N * N;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = ld_cvt<A>(p[i * N + j]);
+}
+
+// Reference SoA gauge [4][2][N*N][S]: forward U_mu(x) at x's parity, backward
+// U_mu(x-mu) at the neighbour (opposite parity), conjugate-transposed here.
+template <int N, class CG, class A>
+__device__ __forceinline__ void load_link_refsoa(cpx<A> (&V)[N][N], const DslashArgs &a, int D, int idx, int nbr)
+{
+  const int mu = D >> 1;
+  const bool bwd = D & 1;
+  const int par = bwd ? 1 - a.parity : a.parity;
+  const int site = bwd ? nbr : idx;
+  const CG *p = static_cast<const CG *>(a.gauge) + size_t(mu * 2 + par) * N * N * a.S + site;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cpx<A> u = ld_cvt<A>(p[size_t(i * N + j) * a.S]);
+      if (bwd) V[j][i] = cpx<A>{u.re, -u.im};
+      else V[i][j] = u;
+    }
+}
+
+template <int D, int N, int R, int SL, int GL, bool LDS, class CS, class CG, class A>
+__device__ __forceinline__ void do_direction(cpx<A> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
+                                             const CG *blk8, A s)
+{
+  if (!g.live[D]) return;
+  cpx<A> V[N][N];
+  if constexpr (GL == GL_NATIVE) load_link_native<N, CG, A>(V, blk8, native_link_slot(D));
+  else load_link_refsoa<N, CG, A>(V, a, D, g.idx, g.nbr[D]);
+  cpx<A> psi[R][4][N];
+  load_spinor<N, R, SL, CS, A>(psi, a, a.in, g.nbr[D], slot);
+#pragma unroll
+  for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V, psi[r], s);
+}
+
+// Sites [first, last] of logical order touched by a workgroup.
+__device__ __forceinline__ uint32_t logical_to_phys(uint32_t sl, const DslashArgs &a, bool natural)
+{
+  if (natural) return sl;
+  const uint32_t row = uint32_t(a.subLx) * a.Ly;
+  uint32_t q = fdiv(sl, a.div_row);
+  uint32_t r = sl - q * row;
+  uint32_t q2 = fdiv(q, a.div_tb);
+  uint32_t tin = q - q2 * a.tblock;
+  uint32_t tout = fdiv(q2, a.div_Lz);
+  uint32_t z = q2 - tout * a.Lz;
+  uint32_t t = tout * a.tblock + tin;
+  return (t * a.Lz + z) * row + r;
+}
+
+// One kernel for every layout / precision / tile combination.
+//   CS: stored input (and xpay operand) complex, CO: stored output complex,
+//   CG: stored link complex, A: accumulation real type.
+//   SITE_MINOR: lanes enumerate sites fastest (reference SoA layout).
+template <int N, int R, int SL, int GL, bool LDS, bool SITE_MINOR, class CS, class CG, class CO, class A>
+__global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
+{
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t blk = xcd_remap(blockIdx.x, a.nblocks);
+  const uint32_t lane0 = blk * blockDim.x;
+  const uint32_t gid = lane0 + threadIdx.x;
+  const bool active = gid < a.nlanes;
+  const bool natural = a.tblock <= 1;
+
+  uint32_t sl, slot;
+  if constexpr (SITE_MINOR) {
+    slot = fdiv(gid, a.div_S);
+    sl = gid - slot * a.S;
+  } else {
+    sl = fdiv(gid, a.div_P);
+    slot = gid - sl * a.P;
+  }
+
+  const CG *blk8 = nullptr;
+  if constexpr (GL == GL_NATIVE && LDS) {
+    static_assert(!SITE_MINOR, "LDS link staging needs site-major lanes");
+    // Stage the 8 links of every site of this workgroup once; all k/R lanes of
+    // a site then read them from LDS (broadcast) instead of from L1/L2.
+    constexpr int chunk16 = (8 * N * N * int(sizeof(CG))) / 16;
+    const uint32_t first = fdiv(lane0, a.div_P);
+    uint32_t lastl = lane0 + blockDim.x - 1;
+    if (lastl >= a.nlanes) lastl = a.nlanes - 1;
+    const uint32_t nsite = fdiv(lastl, a.div_P) - first + 1;
+    uint4 *dst = reinterpret_cast<uint4 *>(smem);
+    const uint4 *src = static_cast<const uint4 *>(a.gauge);
+    for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
+      const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
+      const uint32_t phys = logical_to_phys(first + sloc, a, natural);
+      dst[i] = src[(size_t(a.parity) * a.S + phys) * chunk16 + w];
+    }
+    __syncthreads();
+    if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * N * N;
+  }
+  A nrm[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) nrm[r] = A(0);
+  if (active) {
+    const SiteGeo g = site_geometry(sl, a, natural || SITE_MINOR);
+    if constexpr (GL == GL_NATIVE && !LDS)
+      blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * N * N;
+
+    const A s = A(a.sign);
+    cpx<A> acc[R][4][N];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+        for (int c = 0; c < N; ++c) acc[r][sp][c] = cz<A>();
+
+    do_direction<0, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<1, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<2, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<3, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<4, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<5, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<6, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    do_direction<7, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+
+    // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.
+    A fin[R], fx[R], fo[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int rhs = int(slot) * R + r;
+      fin[r] = A(a.xb) * (a.sc_in ? A(a.sc_in[rhs]) : A(1));
+      fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
+      fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
+    }
+    if constexpr (SL == SL_PACKED) {
+      const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
+      const size_t stride = size_t(a.k) / R;
+      CVec<CO, R> *po = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.out) + base);
+      const CVec<CS, R> *px = a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
+#pragma unroll
+      for (int sc = 0; sc < 4 * N; ++sc) {
+        CVec<CO, R> o;
+        CVec<CS, R> xv;
+        if (px) xv = px[sc * stride];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          cpx<A> v = acc[r][sc / N][sc % N];
+          v.re *= fin[r];
+          v.im *= fin[r];
+          if (px) {
+            const cpx<A> xx = ld_cvt<A>(xv.v[r]);
+            v.re += fx[r] * xx.re;
+            v.im += fx[r] * xx.im;
+          }
+          nrm[r] += v.re * v.re + v.im * v.im;
+          v.re *= fo[r];
+          v.im *= fo[r];
+          o.v[r] = StCvt<CO>::go(v);
+        }
+        po[sc * stride] = o;
+      }
+    } else {
+      CO *po;
+      size_t es;
+      if constexpr (SL == SL_AOS_PTRS) {
+        po = static_cast<CO *>(a.out_ptrs.p[slot]) + size_t(g.idx) * 4 * N;
+        es = 1;
+      } else {
+        po = static_cast<CO *>(a.out_ptrs.p[slot]) + g.idx;
+        es = size_t(a.S);
+      }
+#pragma unroll
+      for (int sc = 0; sc < 4 * N; ++sc) {
+        cpx<A> v = acc[0][sc / N][sc % N];
+        v.re *= fin[0];
+        v.im *= fin[0];
+        nrm[0] += v.re * v.re + v.im * v.im;
+        po[sc * es] = StCvt<CO>::go(v);
+      }
+    }
+  }
+
+  if (a.partial) {
+    // Per-RHS partial sums of |out|^2 for this workgroup (no atomics on
+    // global memory: a second pass reduces the [nblocks][k] partials).
+    __shared__ double red[256];
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.k; i += blockDim.x) red[i] = 0.0;
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) atomicAdd(&red[int(slot) * R + r], double(nrm[r]));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.k; i += blockDim.x) a.partial[size_t(blk) * a.k + i] = red[i];
+  }
+}
+
+}  // namespace qmrhs

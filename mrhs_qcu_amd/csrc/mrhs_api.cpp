@@ -1,0 +1,391 @@
+// Extended C ABI (include/qcu_mrhs.h): packed-field dslash, storage layer and
+// the tile autotuner.
+#include "qcu_mrhs.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "api_internal.h"
+#include "dslash_dispatch.h"
+#include "storage_api.h"
+
+namespace qmrhs {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+int api_fail(const std::string &m)
+{
+  g_last_error = m;
+  return 1;
+}
+
+bool geom_from_c(const qcuMrhsGeom *g, Geom *out, std::string *err)
+{
+  if (!g) {
+    if (err) *err = "null geometry";
+    return false;
+  }
+  for (int d = 0; d < 4; ++d) {
+    out->L[d] = g->L[d];
+    out->grid[d] = g->grid[d] <= 0 ? 1 : g->grid[d];
+    if (g->L[d] <= 0 || (g->L[d] & 1)) {
+      if (err) *err = "lattice extents must be positive and even";
+      return false;
+    }
+  }
+  out->N = g->nColors;
+  if (out->N < 2 || out->N > 8) {
+    if (err) *err = "nColors must be in 2..8";
+    return false;
+  }
+  return true;
+}
+
+int prec_bytes(int prec)
+{
+  switch (prec) {
+    case PREC_DOUBLE: return 16;
+    case PREC_SINGLE: return 8;
+    case PREC_HALF: return 4;
+    case PREC_BF16: return 4;
+    default: return 0;
+  }
+}
+
+long long sites_per_parity(const Geom &g) { return 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3]; }
+
+// ---------------------------------------------------------------------------
+// Tuning cache: (N, nrhs, prec, L, skip mask) -> tile.
+namespace {
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
+std::map<TuneKey, Tile> &tune_cache()
+{
+  static std::map<TuneKey, Tile> m;
+  return m;
+}
+std::mutex &tune_mutex()
+{
+  static std::mutex m;
+  return m;
+}
+TuneKey tune_key(const Geom &g, int prec, int k)
+{
+  int mask = 0;
+  for (int d = 0; d < 4; ++d) mask |= (g.grid[d] > 1 ? 1 : 0) << d;
+  return TuneKey{g.N, k, prec, g.L[0], g.L[1], g.L[2], g.L[3], mask};
+}
+}  // namespace
+
+Tile tile_for(const Geom &g, int prec, int k)
+{
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(tune_key(g, prec, k));
+    if (it != tune_cache().end()) return it->second;
+  }
+  Tile t = default_tile(g.N, k, prec);
+  // Fall back to a plain tile when the preferred one is not instantiated.
+  KernelKey key{SL_PACKED, GL_NATIVE, g.N, t.R, t.lds, 0, prec, prec, prec};
+  if (k % t.R || !find_kernel(key)) {
+    t.R = 1;
+    t.lds = 0;
+  }
+  return t;
+}
+
+void set_tile(const Geom &g, int prec, int k, const Tile &t)
+{
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  tune_cache()[tune_key(g, prec, k)] = t;
+}
+
+std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
+{
+  std::vector<Tile> out;
+  for (int R : {1, 2, 4}) {
+    if (k % R) continue;
+    for (int lds : {0, 1}) {
+      KernelKey key{SL_PACKED, GL_NATIVE, g.N, R, lds, 0, prec, prec, prec};
+      if (!find_kernel(key)) continue;
+      for (int block : {64, 128, 256}) {
+        if (lds) {
+          const int P = k / R;
+          const size_t shm = size_t((block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(prec);
+          if (shm > 65536) continue;
+          if (P < 2) continue;  // no link reuse inside a workgroup
+        }
+        for (int tb : {1, 2, 4}) {
+          if (g.L[3] % tb) continue;
+          Tile t;
+          t.R = R;
+          t.lds = lds;
+          t.block = block;
+          t.tblock = tb;
+          out.push_back(t);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+int run_packed_dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec,
+                      int k, int parity, int dagger, const Geom &g, const Tile *tile, hipStream_t st, std::string *err)
+{
+  DslashCall c;
+  c.sl = SL_PACKED;
+  c.gl = GL_NATIVE;
+  c.prec_s = c.prec_g = c.prec_o = prec;
+  c.geom = g;
+  c.tile = tile ? *tile : tile_for(g, prec, k);
+  c.k = k;
+  c.parity = parity;
+  c.dagger = dagger;
+  c.out = out;
+  c.in = in;
+  c.gauge = gauge;
+  c.x = x;
+  c.xa = xa;
+  c.xb = xb;
+  c.stream = st;
+  return launch_dslash(c, err);
+}
+
+}  // namespace qmrhs
+
+using namespace qmrhs;
+
+extern "C" {
+
+const char *qcuMrhsLastError(void) { return g_last_error.c_str(); }
+
+int qcuMrhsVersion(void) { return 10000; }
+
+size_t qcuMrhsSpinorBytes(const qcuMrhsGeom *gc, int nrhs, int prec)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err) || nrhs < 1 || !prec_bytes(prec)) return 0;
+  return size_t(sites_per_parity(g)) * 4 * g.N * nrhs * prec_bytes(prec);
+}
+
+size_t qcuMrhsGaugeBytes(const qcuMrhsGeom *gc, int prec)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err) || !prec_bytes(prec)) return 0;
+  return size_t(sites_per_parity(g)) * 2 * 8 * g.N * g.N * prec_bytes(prec);
+}
+
+int qcuMrhsPackGauge(void *dst, int dst_prec, const void *src, int src_prec, const qcuMrhsGeom *gc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!dst || !src) return api_fail("null gauge pointer");
+  if (pack_gauge(dst, dst_prec, src, src_prec, g.N, g.L, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsPackSpinors(void *dst, int dst_prec, const void *const *src, int src_prec, int nrhs, const qcuMrhsGeom *gc,
+                       void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!dst || !src || nrhs < 1) return api_fail("null pointer or nrhs < 1");
+  for (int r = 0; r < nrhs; ++r)
+    if (!src[r]) return api_fail("null source spinor");
+  if (pack_spinors(dst, dst_prec, src, src_prec, nrhs, g.N, sites_per_parity(g), static_cast<hipStream_t>(stream),
+                   &err))
+    return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsUnpackSpinors(void *const *dst, int dst_prec, const void *src, int src_prec, int nrhs, const qcuMrhsGeom *gc,
+                         void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!dst || !src || nrhs < 1) return api_fail("null pointer or nrhs < 1");
+  for (int r = 0; r < nrhs; ++r)
+    if (!dst[r]) return api_fail("null destination spinor");
+  if (unpack_spinors(dst, dst_prec, src, src_prec, nrhs, g.N, sites_per_parity(g), static_cast<hipStream_t>(stream),
+                     &err))
+    return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsConvert(void *dst, int dst_prec, const void *src, int src_prec, size_t n, void *stream)
+{
+  std::string err;
+  if (!dst || !src) return api_fail("null pointer");
+  if (convert_complex(dst, dst_prec, src, src_prec, (long long)n, static_cast<hipStream_t>(stream), &err))
+    return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsShiftSpinor(void *dst, const void *src, int direction, const qcuMrhsGeom *gc, int bpc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!dst || !src) return api_fail("null pointer");
+  const int S = int(sites_per_parity(g)), M = 4 * g.N;
+  // TO_COALESCE: [S][M] -> [M][S]; TO_NON_COALESCE: [M][S] -> [S][M].
+  const int rows = direction == 0 ? S : M, cols = direction == 0 ? M : S;
+  if (transpose_batched(dst, src, rows, cols, 1, bpc, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsShiftGauge(void *dst, const void *src, int direction, const qcuMrhsGeom *gc, int bpc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!dst || !src) return api_fail("null pointer");
+  const int S = int(sites_per_parity(g)), M = g.N * g.N;
+  const int rows = direction == 0 ? S : M, cols = direction == 0 ? M : S;
+  if (transpose_batched(dst, src, rows, cols, 8, bpc, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsDslash(void *out, const void *in, const void *gauge, int prec, int nrhs, int parity, int dagger,
+                  const qcuMrhsGeom *gc, const qcuMrhsTile *tc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  Tile t;
+  const Tile *tp = nullptr;
+  if (tc) {
+    t.R = tc->rhs_per_lane;
+    t.block = tc->block;
+    t.lds = tc->link_lds;
+    t.tblock = tc->t_block;
+    tp = &t;
+  }
+  if (run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, parity, dagger, g, tp,
+                        static_cast<hipStream_t>(stream), &err))
+    return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsDslashXpay(void *out, const void *in, const void *x, double a, double b, const void *gauge, int prec,
+                      int nrhs, int parity, int dagger, const qcuMrhsGeom *gc, const qcuMrhsTile *tc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!x) return api_fail("null xpay operand");
+  Tile t;
+  const Tile *tp = nullptr;
+  if (tc) {
+    t.R = tc->rhs_per_lane;
+    t.block = tc->block;
+    t.lds = tc->link_lds;
+    t.tblock = tc->t_block;
+    tp = &t;
+  }
+  if (run_packed_dslash(out, in, x, a, b, gauge, prec, nrhs, parity, dagger, g, tp, static_cast<hipStream_t>(stream),
+                        &err))
+    return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const void *gauge, int prec, int nrhs,
+                      int parity, int dagger, const qcuMrhsGeom *gc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  DslashCall c;
+  c.sl = SL_AOS_PTRS;
+  c.gl = GL_NATIVE;
+  c.prec_s = c.prec_g = c.prec_o = prec;
+  c.geom = g;
+  c.k = nrhs;
+  c.parity = parity;
+  c.dagger = dagger;
+  c.out_ptrs = out_user;
+  c.in_ptrs = in_user;
+  c.gauge = gauge;
+  c.stream = static_cast<hipStream_t>(stream);
+  if (launch_dslash(c, &err)) return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsKernelCount(void) { return registry_size(); }
+
+int qcuMrhsGetTile(const qcuMrhsGeom *gc, int prec, int nrhs, qcuMrhsTile *tile)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!tile) return api_fail("null tile");
+  Tile t = tile_for(g, prec, nrhs);
+  tile->rhs_per_lane = t.R;
+  tile->block = t.block;
+  tile->link_lds = t.lds;
+  tile->t_block = t.tblock;
+  return 0;
+}
+
+int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *gc,
+                qcuMrhsTile *best, double *best_us, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<Tile> cands = tile_candidates(g, prec, nrhs);
+  if (cands.empty()) return api_fail("no instantiated tile for this shape");
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return api_fail("hipEventCreate failed");
+  double best_t = 1e30;
+  Tile best_tile = cands[0];
+  const int reps = 5;
+  for (const Tile &t : cands) {
+    // Warm-up launch also validates the tile.
+    if (run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, 0, 0, g, &t, st, &err)) continue;
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; ++r)
+      run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, r & 1, 0, g, &t, st, &err);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return api_fail("tuning launch failed");
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double us = 1e3 * ms / reps;
+    if (us < best_t) {
+      best_t = us;
+      best_tile = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  set_tile(g, prec, nrhs, best_tile);
+  if (best) {
+    best->rhs_per_lane = best_tile.R;
+    best->block = best_tile.block;
+    best->link_lds = best_tile.lds;
+    best->t_block = best_tile.tblock;
+  }
+  if (best_us) *best_us = best_t;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,26 @@
+// Host entry points of the storage kernels (all asynchronous on `st`;
+// 0 on success, otherwise 1 with a message in *err).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace qmrhs {
+
+// dst[b][c][r] = src[b][r][c] for b < nb, rows x cols elements of elem_bytes.
+int transpose_batched(void *dst, const void *src, int rows, int cols, int nb, int elem_bytes, hipStream_t st,
+                      std::string *err);
+// k user spinors [S][4][N] (host array of device pointers) -> packed [S][4][N][k].
+int pack_spinors(void *dst, int dst_prec, const void *const *src, int src_prec, int k, int N, long long S,
+                 hipStream_t st, std::string *err);
+// packed [S][4][N][k] -> k user spinors.
+int unpack_spinors(void *const *dst, int dst_prec, const void *src, int src_prec, int k, int N, long long S,
+                   hipStream_t st, std::string *err);
+// user gauge [4][2][S][N][N] (FP64 or FP32) -> native [2][S][8][N][N] in dst_prec.
+int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, const int *L, hipStream_t st,
+               std::string *err);
+// n complex numbers, element-wise precision conversion.
+int convert_complex(void *dst, int dst_prec, const void *src, int src_prec, long long n, hipStream_t st,
+                    std::string *err);
+
+}  // namespace qmrhs

@@ -1,0 +1,375 @@
+// Storage layer: layout changes between the user layouts, the reference SoA
+// layout and the native RHS-innermost layouts, with precision conversion.
+//
+//  * shift kernels (reference src/storage/qcu_storage.cu:9-35 and
+//    include/kernels/storage/storage_kernels.cuh:14-101): pure permutations,
+//    spinor [S][4N] <-> [4N][S] and gauge [4][2][S][N*N] <-> [4][2][N*N][S],
+//    done here as one LDS-tiled batched transpose of E-byte elements so both
+//    the reads and the writes are coalesced.
+//  * pack / unpack: k user spinors [S][4][N] (one pointer each) <-> the native
+//    packed field [S][4][N][k].
+//  * gauge pack: user gauge [4][2][S][N][N] -> native [2][S][8][N][N] holding,
+//    per site x of parity p, U_mu(x) (slots 0..3) and U_mu(x-mu)^+ (slots 4..7).
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <type_traits>
+
+#include "qcu_device.h"
+#include "storage_api.h"
+
+namespace qmrhs {
+
+// ---------------------------------------------------------------------------
+// Batched transpose: for b < nb: dst[b][c][r] = src[b][r][c], rows x cols
+// elements of type T (T = the element's bytes: uint32_t / uint2 / uint4).
+template <class T>
+__global__ void __launch_bounds__(256) transpose_kernel(T *__restrict__ dst, const T *__restrict__ src, int rows,
+                                                        int cols, int nb)
+{
+  __shared__ T tile[32][33];
+  const int b = blockIdx.z;
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const T *s = src + size_t(b) * rows * cols;
+  T *d = dst + size_t(b) * rows * cols;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = s[size_t(r) * cols + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) d[size_t(c) * rows + r] = tile[tx][j];
+  }
+  (void)nb;
+}
+
+int transpose_batched(void *dst, const void *src, int rows, int cols, int nb, int elem_bytes, hipStream_t st,
+                      std::string *err)
+{
+  if (rows <= 0 || cols <= 0 || nb <= 0) {
+    if (err) *err = "empty transpose";
+    return 1;
+  }
+  if (nb > 65535) {
+    if (err) *err = "too many transpose batches";
+    return 1;
+  }
+  dim3 grid((rows + 31) / 32, (cols + 31) / 32, nb), block(256);
+  if (grid.y > 65535) {
+    if (err) *err = "transpose too wide";
+    return 1;
+  }
+  switch (elem_bytes) {
+    case 4:
+      hipLaunchKernelGGL(transpose_kernel<uint32_t>, grid, block, 0, st, static_cast<uint32_t *>(dst),
+                         static_cast<const uint32_t *>(src), rows, cols, nb);
+      break;
+    case 8:
+      hipLaunchKernelGGL(transpose_kernel<uint2>, grid, block, 0, st, static_cast<uint2 *>(dst),
+                         static_cast<const uint2 *>(src), rows, cols, nb);
+      break;
+    case 16:
+      hipLaunchKernelGGL(transpose_kernel<uint4>, grid, block, 0, st, static_cast<uint4 *>(dst),
+                         static_cast<const uint4 *>(src), rows, cols, nb);
+      break;
+    default:
+      if (err) *err = "element size must be 4, 8 or 16 bytes";
+      return 1;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    if (err) *err = std::string("transpose launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Precision-converting element copy helpers.
+template <class CD, class CSrc>
+__device__ __forceinline__ CD cvt_elem(const CSrc &v)
+{
+  if constexpr (sizeof(CD) == 16 && sizeof(CSrc) == 16) {
+    return v;  // FP64 -> FP64: bit copy
+  } else {
+    using A = typename std::conditional<sizeof(CD) == 16, double, float>::type;
+    if constexpr (std::is_same<CD, c16>::value && std::is_same<CSrc, c64>::value) {
+      // Direct double -> binary16 rounding (one rounding step).
+      c16 r;
+      r.x = _Float16(v.x);
+      r.y = _Float16(v.y);
+      return r;
+    } else {
+      return StCvt<CD>::go(ld_cvt<A>(v));
+    }
+  }
+}
+
+struct PtrTable {
+  void *p[64];
+};
+
+// Pack: dst[((s*4N)+e)*k + r] = cvt(src_r[s*4N + e]); one lane per element.
+template <class CD, class CSrc>
+__global__ void __launch_bounds__(256) pack_kernel(CD *__restrict__ dst, PtrTable src, int r0, int kk, int k,
+                                                   int M /*4N*/, long long nelem /* S*M*kk */)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nelem) return;
+  const int r = int(i % kk);
+  const long long se = i / kk;  // s*M + e
+  const CSrc *s = static_cast<const CSrc *>(src.p[r]);
+  dst[se * k + r0 + r] = cvt_elem<CD, CSrc>(s[se]);
+  (void)M;
+}
+
+template <class CD, class CSrc>
+__global__ void __launch_bounds__(256) unpack_kernel(PtrTable dst, const CSrc *__restrict__ src, int r0, int kk, int k,
+                                                     long long nelem)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nelem) return;
+  const int r = int(i % kk);
+  const long long se = i / kk;
+  CD *d = static_cast<CD *>(dst.p[r]);
+  d[se] = cvt_elem<CD, CSrc>(src[se * k + r0 + r]);
+}
+
+template <class CD, class CSrc>
+static hipError_t launch_pack(bool unpack, void *packed, void *const *ptrs, int k, int M, long long S, hipStream_t st)
+{
+  for (int r0 = 0; r0 < k; r0 += 64) {
+    const int kk = (k - r0) < 64 ? (k - r0) : 64;
+    PtrTable t{};
+    for (int r = 0; r < kk; ++r) t.p[r] = ptrs[r0 + r];
+    const long long nelem = S * M * kk;
+    const unsigned nblk = unsigned((nelem + 255) / 256);
+    if (!unpack)
+      hipLaunchKernelGGL((pack_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(packed), t, r0, kk,
+                         k, M, nelem);
+    else
+      hipLaunchKernelGGL((unpack_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, t,
+                         static_cast<const CSrc *>(packed), r0, kk, k, nelem);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <class CD>
+static hipError_t pack_dispatch_src(bool unpack, int src_prec, void *packed, void *const *ptrs, int k, int M,
+                                    long long S, hipStream_t st, bool *ok)
+{
+  *ok = true;
+  // For pack: CD is the packed (destination) type, the source is the user type.
+  // For unpack: CD is the user (destination) type, the source is the packed type.
+  switch (src_prec) {
+    case PREC_DOUBLE: return launch_pack<CD, c64>(unpack, packed, ptrs, k, M, S, st);
+    case PREC_SINGLE: return launch_pack<CD, c32>(unpack, packed, ptrs, k, M, S, st);
+    case PREC_HALF: return launch_pack<CD, c16>(unpack, packed, ptrs, k, M, S, st);
+    case PREC_BF16: return launch_pack<CD, cb16>(unpack, packed, ptrs, k, M, S, st);
+    default: *ok = false; return hipSuccess;
+  }
+}
+
+int pack_spinors(void *dst, int dst_prec, const void *const *src, int src_prec, int k, int N, long long S,
+                 hipStream_t st, std::string *err)
+{
+  bool ok = false;
+  hipError_t e = hipSuccess;
+  void *const *ptrs = const_cast<void *const *>(src);
+  const int M = 4 * N;
+  switch (dst_prec) {
+    case PREC_DOUBLE: e = pack_dispatch_src<c64>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
+    case PREC_SINGLE: e = pack_dispatch_src<c32>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
+    case PREC_HALF: e = pack_dispatch_src<c16>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
+    case PREC_BF16: e = pack_dispatch_src<cb16>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
+    default: ok = false;
+  }
+  if (!ok) {
+    if (err) *err = "unsupported precision";
+    return 1;
+  }
+  if (e != hipSuccess) {
+    if (err) *err = std::string("pack launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+int unpack_spinors(void *const *dst, int dst_prec, const void *src, int src_prec, int k, int N, long long S,
+                   hipStream_t st, std::string *err)
+{
+  bool ok = false;
+  hipError_t e = hipSuccess;
+  void *packed = const_cast<void *>(src);
+  const int M = 4 * N;
+  switch (dst_prec) {
+    case PREC_DOUBLE: e = pack_dispatch_src<c64>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
+    case PREC_SINGLE: e = pack_dispatch_src<c32>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
+    case PREC_HALF: e = pack_dispatch_src<c16>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
+    case PREC_BF16: e = pack_dispatch_src<cb16>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
+    default: ok = false;
+  }
+  if (!ok) {
+    if (err) *err = "unsupported precision";
+    return 1;
+  }
+  if (e != hipSuccess) {
+    if (err) *err = std::string("unpack launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Gauge: user [4][2][S][N][N] -> native [2][S][8][N][N]. One lane per
+// (parity, site, slot, element).
+template <class CD, class CSrc>
+__global__ void __launch_bounds__(256) gauge_pack_kernel(CD *__restrict__ dst, const CSrc *__restrict__ src, int N,
+                                                         int subLx, int Ly, int Lz, int Lt, long long nelem)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nelem) return;
+  const int NN = N * N;
+  const long long S = 1LL * subLx * Ly * Lz * Lt;
+  const int e = int(i % NN);
+  const long long q = i / NN;
+  const int slot = int(q % 8);
+  const long long ps = q / 8;
+  const int p = int(ps / S);
+  const int s = int(ps - 1LL * p * S);
+  const int row = e / N, col = e % N;
+  if (slot < 4) {
+    dst[i] = cvt_elem<CD, CSrc>(src[((size_t(slot) * 2 + p) * S + s) * NN + e]);
+    return;
+  }
+  // Backward link of direction mu: U_mu(x - mu)^+, stored at x - mu (parity 1-p).
+  const int mu = slot - 4;
+  int xh = s % subLx;
+  int r = s / subLx;
+  const int y = r % Ly;
+  r /= Ly;
+  const int z = r % Lz;
+  const int t = r / Lz;
+  const int eo = (y + z + t) & 1;
+  int nx = xh, ny = y, nz = z, nt = t;
+  if (mu == 0) nx = (eo == p) ? (xh == 0 ? subLx - 1 : xh - 1) : xh;
+  if (mu == 1) ny = (y == 0) ? Ly - 1 : y - 1;
+  if (mu == 2) nz = (z == 0) ? Lz - 1 : z - 1;
+  if (mu == 3) nt = (t == 0) ? Lt - 1 : t - 1;
+  const long long nidx = ((1LL * nt * Lz + nz) * Ly + ny) * subLx + nx;
+  CSrc u = src[((size_t(mu) * 2 + (1 - p)) * S + nidx) * NN + col * N + row];
+  u.y = -u.y;
+  dst[i] = cvt_elem<CD, CSrc>(u);
+}
+
+template <class CD>
+static hipError_t gauge_pack_src(int src_prec, void *dst, const void *src, int N, const int *L, hipStream_t st,
+                                 bool *ok)
+{
+  const long long S = 1LL * L[0] / 2 * L[1] * L[2] * L[3];
+  const long long nelem = 2LL * S * 8 * N * N;
+  const unsigned nblk = unsigned((nelem + 255) / 256);
+  *ok = true;
+  switch (src_prec) {
+    case PREC_DOUBLE:
+      hipLaunchKernelGGL((gauge_pack_kernel<CD, c64>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c64 *>(src), N, L[0] / 2, L[1], L[2], L[3], nelem);
+      break;
+    case PREC_SINGLE:
+      hipLaunchKernelGGL((gauge_pack_kernel<CD, c32>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c32 *>(src), N, L[0] / 2, L[1], L[2], L[3], nelem);
+      break;
+    default: *ok = false; return hipSuccess;
+  }
+  return hipGetLastError();
+}
+
+int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, const int *L, hipStream_t st,
+               std::string *err)
+{
+  bool ok = false;
+  hipError_t e = hipSuccess;
+  switch (dst_prec) {
+    case PREC_DOUBLE: e = gauge_pack_src<c64>(src_prec, dst, src, N, L, st, &ok); break;
+    case PREC_SINGLE: e = gauge_pack_src<c32>(src_prec, dst, src, N, L, st, &ok); break;
+    case PREC_HALF: e = gauge_pack_src<c16>(src_prec, dst, src, N, L, st, &ok); break;
+    case PREC_BF16: e = gauge_pack_src<cb16>(src_prec, dst, src, N, L, st, &ok); break;
+    default: ok = false;
+  }
+  if (!ok) {
+    if (err) *err = "unsupported gauge precision (source must be FP64 or FP32)";
+    return 1;
+  }
+  if (e != hipSuccess) {
+    if (err) *err = std::string("gauge pack launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise precision conversion of a contiguous array of complex numbers.
+template <class CD, class CSrc>
+__global__ void __launch_bounds__(256) convert_kernel(CD *__restrict__ dst, const CSrc *__restrict__ src,
+                                                      long long n)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i < n) dst[i] = cvt_elem<CD, CSrc>(src[i]);
+}
+
+template <class CD>
+static hipError_t convert_src(int src_prec, void *dst, const void *src, long long n, hipStream_t st, bool *ok)
+{
+  const unsigned nblk = unsigned((n + 255) / 256);
+  *ok = true;
+  switch (src_prec) {
+    case PREC_DOUBLE:
+      hipLaunchKernelGGL((convert_kernel<CD, c64>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c64 *>(src), n);
+      break;
+    case PREC_SINGLE:
+      hipLaunchKernelGGL((convert_kernel<CD, c32>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c32 *>(src), n);
+      break;
+    case PREC_HALF:
+      hipLaunchKernelGGL((convert_kernel<CD, c16>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c16 *>(src), n);
+      break;
+    case PREC_BF16:
+      hipLaunchKernelGGL((convert_kernel<CD, cb16>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const cb16 *>(src), n);
+      break;
+    default: *ok = false; return hipSuccess;
+  }
+  return hipGetLastError();
+}
+
+int convert_complex(void *dst, int dst_prec, const void *src, int src_prec, long long n, hipStream_t st,
+                    std::string *err)
+{
+  bool ok = false;
+  hipError_t e = hipSuccess;
+  if (n <= 0) return 0;
+  switch (dst_prec) {
+    case PREC_DOUBLE: e = convert_src<c64>(src_prec, dst, src, n, st, &ok); break;
+    case PREC_SINGLE: e = convert_src<c32>(src_prec, dst, src, n, st, &ok); break;
+    case PREC_HALF: e = convert_src<c16>(src_prec, dst, src, n, st, &ok); break;
+    case PREC_BF16: e = convert_src<cb16>(src_prec, dst, src, n, st, &ok); break;
+    default: ok = false;
+  }
+  if (!ok) {
+    if (err) *err = "unsupported precision";
+    return 1;
+  }
+  if (e != hipSuccess) {
+    if (err) *err = std::string("convert launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+}  // namespace qmrhs

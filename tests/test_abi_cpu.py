@@ -1,0 +1,117 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the public
+headers declare, validates arguments without touching the GPU, and keeps the
+reference's fatal-error convention (message on stderr, exit status 1)."""
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//.*", "", text)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|size_t|char)\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol(lib):
+    from mrhs_qcu_amd import _lib
+
+    for header, listed in (("qcu.h", _lib.QCU_H_SYMBOLS), ("qcu_mrhs.h", _lib.QCU_MRHS_H_SYMBOLS)):
+        names = declared_functions(header)
+        assert sorted(names) == sorted(listed), (header, names)
+        for n in names:
+            assert hasattr(lib, n), n
+
+
+def test_kernel_registry(lib):
+    assert lib.qcuMrhsKernelCount() >= 120
+    assert lib.qcuMrhsVersion() >= 10000
+
+
+def test_sizes(lib):
+    from mrhs_qcu_amd import _lib
+
+    g = _lib.geom((24, 24, 24, 48), 3)
+    S = 12 * 24 * 24 * 48
+    assert lib.qcuMrhsSpinorBytes(g, 16, _lib.SINGLE) == S * 12 * 16 * 8
+    assert lib.qcuMrhsGaugeBytes(g, _lib.HALF) == S * 2 * 8 * 9 * 4
+    bad = _lib.geom((24, 24, 23, 48), 3)
+    assert lib.qcuMrhsSpinorBytes(bad, 16, _lib.SINGLE) == 0
+
+
+def test_argument_validation_without_gpu(lib):
+    from mrhs_qcu_amd import _lib
+
+    t = _lib.tile()
+    cases = [
+        (_lib.geom((4, 4, 4, 5), 3), 2, "even"),
+        (_lib.geom((4, 4, 4, 4), 9), 2, "nColors"),
+        (_lib.geom((4, 4, 4, 4), 3), 0, "nrhs"),
+    ]
+    for g, k, msg in cases:
+        rc = lib.qcuMrhsDslash(16, 16, 16, _lib.SINGLE, k, 0, 0, g, t, None)
+        assert rc != 0
+        assert msg in lib.qcuMrhsLastError().decode()
+    g = _lib.geom((4, 4, 4, 4), 3)
+    assert lib.qcuMrhsDslash(16, 16, 16, _lib.SINGLE, 3, 0, 0, g, _lib.tile(R=2), None) != 0
+    assert "multiple" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash(16, 16, 16, _lib.SINGLE, 2, 0, 0, g, _lib.tile(block=96), None) != 0
+    assert lib.qcuMrhsDslash(8, 16, 16, _lib.SINGLE, 2, 0, 0, g, _lib.tile(), None) != 0
+    assert "aligned" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash(16, 16, 16, 3, 2, 0, 0, g, _lib.tile(), None) != 0
+    assert lib.qcuMrhsDslash(16, 16, 16, _lib.SINGLE, 2, 2, 0, g, _lib.tile(), None) != 0
+
+
+DEATH = [
+    ("lib.startDslash(0, 0)", "Qcu is not initialized."),
+    ("lib.pushBackFermions(1, 1)", "Qcu is not initialized."),
+    ("lib.getDslash(0, 0.0, 3, 1, 2)", "Qcu is not initialized."),
+    ("lib.loadQcuGauge(1)", "Qcu is not initialized."),
+    ("init(3); lib.getDslash(0, 0.0, 3, 1, 2)", "localGauge_ is nullptr, not loaded."),
+    ("init(3); lib.startDslash(0, 0)", "Dslash operator not created"),
+    ("init(3); lib.pushBackFermions(0, 0)", "fermion pointer is null."),
+    ("init(9)", "Now we only support 2-8 colors."),
+    ("init(3, L=(4, 4, 3, 4))", "lattice sizes must be positive and even."),
+]
+
+
+@pytest.mark.parametrize("code,message", DEATH)
+def test_fatal_errors_exit_1(code, message):
+    script = (
+        "import ctypes, sys\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "from mrhs_qcu_amd import _lib\n"
+        "lib = _lib.load()\n"
+        "def init(n, L=(4, 4, 4, 4)):\n"
+        "    g = _lib.QcuGrid(); p = _lib.QcuParam()\n"
+        "    for d in range(4):\n"
+        "        g.grid_size[d] = 1; p.lattice_size[d] = L[d]\n"
+        "    lib.initGridSize(ctypes.byref(g), ctypes.byref(p), n)\n"
+        f"{code}\n"
+        "print('returned')\n"
+    )
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+    assert message in r.stderr
+    assert "returned" not in r.stdout
+
+
+def test_deprecated_entry_points_are_noops(lib):
+    lib.dslashQcu(None, None, None, None, 0)
+    lib.fullDslashQcu(None, None, None, None, 0)
+
+
+def test_fastdiv_exact(tmp_path):
+    exe = tmp_path / "fastdiv_check"
+    src = os.path.join(ROOT, "tests", "cpp", "fastdiv_check.cpp")
+    inc = os.path.join(ROOT, "mrhs_qcu_amd", "csrc")
+    subprocess.run(["hipcc", "-O2", "-std=c++17", "-I", inc, src, "-o", str(exe)], check=True, timeout=300,
+                   capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout

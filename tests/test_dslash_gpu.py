@@ -1,0 +1,304 @@
+"""GPU parity of the dslash paths against the FP64 oracle.
+
+Tolerances (relative error per RHS, max over RHS, vs the FP64 oracle):
+FP64 1e-13, FP32 1e-6; FP16 / BF16 storage with FP32 arithmetic are stated
+(printed) and bounded loosely (3e-3 / 2e-2).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from mrhs_qcu_amd import _lib, fields
+from oracle import oracle
+from helpers import rel_err, dev_tensor, host_complex, empty_prec, native_gauge
+
+pytestmark = pytest.mark.gpu
+
+TOL = {2: 1e-13, 1: 1e-6, 0: 3e-3, 4: 2e-2}
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def pack_inputs(lib, dev, g_user, psi_user, L, N, prec):
+    geom = _lib.geom(L, N)
+    gd = dev_tensor(g_user, dev)
+    gnat = empty_prec((lib.qcuMrhsGaugeBytes(geom, prec) // _lib.BYTES[prec],), prec, dev)
+    _lib.check(lib.qcuMrhsPackGauge(gnat.data_ptr(), prec, gd.data_ptr(), _lib.DOUBLE, geom, stream()))
+    k = psi_user.shape[0]
+    ins = [dev_tensor(psi_user[r], dev) for r in range(k)]
+    arr, keep = _lib.ptr_array([t.data_ptr() for t in ins])
+    pin = empty_prec(psi_user.shape[1:] + (k,), prec, dev)
+    _lib.check(lib.qcuMrhsPackSpinors(pin.data_ptr(), prec, arr, _lib.DOUBLE, k, geom, stream()))
+    return geom, gnat, pin
+
+
+def unpack(lib, dev, packed, prec, k, geom, shape):
+    outs = [torch.empty(shape + (2,), dtype=torch.float64, device=dev) for _ in range(k)]
+    arr, keep = _lib.ptr_array([t.data_ptr() for t in outs])
+    _lib.check(lib.qcuMrhsUnpackSpinors(arr, _lib.DOUBLE, packed.data_ptr(), prec, k, geom, stream()))
+    torch.cuda.synchronize()
+    return np.stack([host_complex(t) for t in outs])
+
+
+def run_packed(lib, dev, g, psi, L, N, parity, dagger, prec, tile=None, grid=(1, 1, 1, 1)):
+    k = psi.shape[0]
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+    for d in range(4):
+        geom.grid[d] = grid[d]
+    pout = torch.empty_like(pin)
+    _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger, geom,
+                                 tile, stream()))
+    return unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+
+
+def test_config1_fp64_8x8x8x8(lib, dev):
+    """8^4, SU(3), nrhs=1, FP64, both parities and dagger flags: <= 1e-13."""
+    L, N = (8, 8, 8, 8), 3
+    g = fields.gauge_field(L, N, seed=0)
+    psi = fields.gaussian_spinors(L, N, 1, seed=0)
+    for parity in (0, 1):
+        for dagger in (0, 1):
+            got = run_packed(lib, dev, g, psi, L, N, parity, dagger, _lib.DOUBLE)
+            want = oracle.dslash(g, psi, L, parity, dagger)
+            err = rel_err(got, want)
+            print(f"c1 fp64 parity={parity} dagger={dagger} rel_err={err:.3e}")
+            assert err < TOL[2]
+
+
+def test_config2_fp32_16x16x16x32_k12(lib, dev):
+    """16^3x32, SU(3), nrhs=12, FP32 vs FP64 oracle: <= 1e-6, both parities."""
+    L, N, k = (16, 16, 16, 32), 3, 12
+    g = fields.gauge_field(L, N, seed=0)
+    psi = fields.gaussian_spinors(L, N, k, seed=0)
+    for parity, dagger in ((0, 0), (1, 0), (0, 1)):
+        got = run_packed(lib, dev, g, psi, L, N, parity, dagger, _lib.SINGLE)
+        want = oracle.dslash(g, psi, L, parity, dagger)
+        err = rel_err(got, want)
+        print(f"c2 fp32 parity={parity} dagger={dagger} rel_err={err:.3e}")
+        assert err < TOL[1]
+
+
+@pytest.mark.parametrize("prec", [_lib.HALF, _lib.BF16])
+def test_low_precision_storage_error_stated(lib, dev, prec):
+    """FP16 / BF16 storage, FP32 arithmetic, smeared links (eps = 0.3)."""
+    L, N, k = (8, 8, 8, 16), 3, 16
+    g = fields.gauge_field(L, N, seed=0, smear_eps=0.3)
+    psi = fields.gaussian_spinors(L, N, k, seed=0)
+    got = run_packed(lib, dev, g, psi, L, N, 0, 0, prec)
+    want = oracle.dslash(g, psi, L, 0, 0)
+    err = rel_err(got, want)
+    print(f"{'fp16' if prec == 0 else 'bf16'} storage rel_err vs fp64 oracle = {err:.3e}")
+    assert err < TOL[prec]
+
+
+@pytest.mark.parametrize("N", [2, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("prec", [_lib.DOUBLE, _lib.SINGLE])
+def test_colors(lib, dev, N, prec):
+    L = (4, 4, 4, 8)
+    k = 4
+    g = fields.gauge_field(L, N, seed=N)
+    psi = fields.gaussian_spinors(L, N, k, seed=N + 1)
+    for parity in (0, 1):
+        got = run_packed(lib, dev, g, psi, L, N, parity, 1, prec)
+        want = oracle.dslash(g, psi, L, parity, 1)
+        assert rel_err(got, want) < TOL[prec]
+
+
+def test_every_tile_agrees(lib, dev):
+    """Every instantiated tile (R, LDS staging, block, t-blocking) gives the
+    oracle's result (FP32, N = 3, k = 8 and k = 6)."""
+    L, N = (8, 4, 6, 8), 3
+    g = fields.gauge_field(L, N, seed=3)
+    for k in (8, 6):
+        psi = fields.gaussian_spinors(L, N, k, seed=4)
+        want = oracle.dslash(g, psi, L, 1, 0)
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+        n = 0
+        for R in (1, 2, 4):
+            for lds in (0, 1):
+                for block in (64, 128, 192, 256):
+                    for tb in (1, 2, 4):
+                        if k % R or L[3] % tb:
+                            continue
+                        pout = torch.zeros_like(pin)
+                        rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 1, 0,
+                                               geom, _lib.tile(R, block, lds, tb), stream())
+                        if rc != 0:
+                            msg = lib.qcuMrhsLastError().decode()
+                            assert "no kernel instantiated" in msg or "LDS" in msg, msg
+                            continue
+                        got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+                        assert rel_err(got, want) < TOL[1], (R, lds, block, tb)
+                        n += 1
+        assert n >= 24
+
+
+@pytest.mark.parametrize("grid", [(2, 1, 1, 1), (1, 1, 1, 2), (2, 2, 2, 2)])
+def test_boundary_skip(lib, dev, grid):
+    L, N, k = (4, 4, 4, 4), 3, 2
+    g = fields.gauge_field(L, N, seed=9)
+    psi = fields.gaussian_spinors(L, N, k, seed=10)
+    for parity in (0, 1):
+        got = run_packed(lib, dev, g, psi, L, N, parity, 0, _lib.DOUBLE, grid=grid)
+        want = oracle.dslash(g, psi, L, parity, 0, grid=grid)
+        assert rel_err(got, want) < TOL[2]
+
+
+def test_xpay(lib, dev):
+    L, N, k = (4, 4, 4, 8), 3, 4
+    g = fields.gauge_field(L, N, seed=11)
+    psi = fields.gaussian_spinors(L, N, k, seed=12)
+    x = fields.gaussian_spinors(L, N, k, seed=13)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.DOUBLE)
+    _, _, px = pack_inputs(lib, dev, g, x, L, N, _lib.DOUBLE)
+    pout = torch.empty_like(pin)
+    a, b = 1.0, -0.135 ** 2
+    _lib.check(lib.qcuMrhsDslashXpay(pout.data_ptr(), pin.data_ptr(), px.data_ptr(), a, b, gnat.data_ptr(),
+                                     _lib.DOUBLE, k, 0, 1, geom, None, stream()))
+    got = unpack(lib, dev, pout, _lib.DOUBLE, k, geom, psi.shape[1:])
+    want = a * x + b * oracle.dslash(g, psi, L, 0, 1)
+    assert rel_err(got, want) < TOL[2]
+
+
+def test_pack_roundtrip_and_gauge_layout(lib, dev):
+    L, N, k = (4, 4, 6, 4), 3, 5
+    g = fields.gauge_field(L, N, seed=14)
+    psi = fields.gaussian_spinors(L, N, k, seed=15)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.DOUBLE)
+    # packed layout [S][4][N][k], bit-exact
+    assert np.array_equal(host_complex(pin), fields.to_packed(psi))
+    # native gauge layout, bit-exact
+    assert np.array_equal(host_complex(gnat).reshape(2, -1, 8, N, N), native_gauge(g, L))
+    # round trip, bit-exact
+    back = unpack(lib, dev, pin, _lib.DOUBLE, k, geom, psi.shape[1:])
+    assert np.array_equal(back, psi)
+    # FP32 / FP16 packing rounds to nearest even (numpy's casts)
+    for prec, npt in ((_lib.SINGLE, np.float32), (_lib.HALF, np.float16)):
+        _, _, p = pack_inputs(lib, dev, g, psi, L, N, prec)
+        host = p.detach().cpu().numpy()
+        want = fields.to_packed(psi)
+        assert np.array_equal(host[..., 0], want.real.astype(npt))
+        assert np.array_equal(host[..., 1], want.imag.astype(npt))
+
+
+def test_user_layout_path(lib, dev):
+    """qcuMrhsDslashUser: user AoS spinors in place, native gauge."""
+    L, N, k = (8, 4, 4, 8), 3, 5
+    g = fields.gauge_field(L, N, seed=16)
+    psi = fields.gaussian_spinors(L, N, k, seed=17)
+    for prec in (_lib.DOUBLE, _lib.SINGLE):
+        geom, gnat, _ = pack_inputs(lib, dev, g, psi[:1], L, N, prec)
+        cdt = np.complex128 if prec == _lib.DOUBLE else np.complex64
+        ins = [dev_tensor(psi[r].astype(cdt), dev) for r in range(k)]
+        outs = [torch.empty_like(t) for t in ins]
+        ai, k1 = _lib.ptr_array([t.data_ptr() for t in ins])
+        ao, k2 = _lib.ptr_array([t.data_ptr() for t in outs])
+        _lib.check(lib.qcuMrhsDslashUser(ao, ai, gnat.data_ptr(), prec, k, 1, 0, geom, stream()))
+        torch.cuda.synchronize()
+        got = np.stack([host_complex(t) for t in outs])
+        assert rel_err(got, oracle.dslash(g, psi, L, 1, 0)) < TOL[prec]
+
+
+def test_reference_layout_path(lib, dev):
+    """qcu::instantiateDslash on reference SoA spinors and SoA gauge."""
+    L, N, k = (4, 4, 4, 8), 4, 3
+    g = fields.gauge_field(L, N, seed=18)
+    psi = fields.gaussian_spinors(L, N, k, seed=19)
+    S = fields.half_volume(L)
+    gsoa = oracle.shift_gauge(g.reshape(-1), True, S, N)
+    gd = dev_tensor(gsoa, dev)
+    ins = [dev_tensor(oracle.shift_vector(psi[r].reshape(-1), True, S, N), dev) for r in range(k)]
+    outs = [torch.empty_like(t) for t in ins]
+    ai, k1 = _lib.ptr_array([t.data_ptr() for t in ins])
+    ao, k2 = _lib.ptr_array([t.data_ptr() for t in outs])
+    geom = _lib.geom(L, N)
+    _lib.check(lib.qcuMrhsDslashReferenceLayout(ao, ai, gd.data_ptr(), _lib.DOUBLE, k, 0, 1, geom))
+    got = np.stack([oracle.shift_vector(host_complex(t), False, S, N).reshape(S, 4, N) for t in outs])
+    assert rel_err(got, oracle.dslash(g, psi, L, 0, 1)) < TOL[2]
+
+
+def test_shift_kernels_bit_exact(lib, dev):
+    L, N = (6, 4, 4, 4), 3
+    S = fields.half_volume(L)
+    v = fields.gaussian_spinors(L, N, 1, seed=20)[0].reshape(-1)
+    g = fields.gauge_field(L, N, seed=21).reshape(-1)
+    geom = _lib.geom(L, N)
+    vd, gd = dev_tensor(v, dev), dev_tensor(g, dev)
+    vs, gs = torch.empty_like(vd), torch.empty_like(gd)
+    _lib.check(lib.qcuMrhsShiftSpinor(vs.data_ptr(), vd.data_ptr(), 0, geom, 16, stream()))
+    _lib.check(lib.qcuMrhsShiftGauge(gs.data_ptr(), gd.data_ptr(), 0, geom, 16, stream()))
+    torch.cuda.synchronize()
+    assert np.array_equal(host_complex(vs), oracle.shift_vector(v, True, S, N))
+    assert np.array_equal(host_complex(gs), oracle.shift_gauge(g, True, S, N))
+    vb, gb = torch.empty_like(vd), torch.empty_like(gd)
+    _lib.check(lib.qcuMrhsShiftSpinor(vb.data_ptr(), vs.data_ptr(), 1, geom, 16, stream()))
+    _lib.check(lib.qcuMrhsShiftGauge(gb.data_ptr(), gs.data_ptr(), 1, geom, 16, stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(vb, vd) and torch.equal(gb, gd)
+
+
+def test_qcu_h_call_sequence(lib, dev):
+    """The reference's src/test.cu:39-48 sequence with seeded data, k = 3,
+    FP64 and FP32 buffers: initGridSize -> loadQcuGauge -> getDslash ->
+    pushBackFermions x k -> startDslash."""
+    L, N, k = (8, 8, 8, 8), 3, 3
+    g = fields.gauge_field(L, N, seed=0)
+    psi = fields.gaussian_spinors(L, N, k, seed=22)
+    grid = _lib.QcuGrid()
+    param = _lib.QcuParam()
+    for d in range(4):
+        grid.grid_size[d] = 1
+        param.lattice_size[d] = L[d]
+    lib.initGridSize(ctypes.byref(grid), ctypes.byref(param), N)
+    gd = dev_tensor(g, dev)
+    lib.loadQcuGauge(gd.data_ptr())
+    for prec, cdt in ((2, np.complex128), (1, np.complex64)):
+        lib.getDslash(0, -3.5, N, k, prec)
+        ins = [dev_tensor(psi[r].astype(cdt), dev) for r in range(k)]
+        outs = [torch.full_like(t, float("nan")) for t in ins]
+        torch.cuda.synchronize()
+        for r in range(k):
+            lib.pushBackFermions(outs[r].data_ptr(), ins[r].data_ptr())
+        # inputs are consumed at push time
+        for t in ins:
+            t.zero_()
+        torch.cuda.synchronize()
+        for parity in (0,):
+            lib.startDslash(parity, 0)
+            got = np.stack([host_complex(t) for t in outs])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, 0)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [_lib.SINGLE, _lib.HALF])
+def test_full_size_properties(lib, dev, prec):
+    """24^3x48, nrhs = 16: RHS independence (a permutation of the RHS permutes
+    the output bit for bit) and linearity in the input."""
+    L, N, k = (24, 24, 24, 48), 3, 16
+    geom = _lib.geom(L, N)
+    S = fields.half_volume(L)
+    torch.manual_seed(0)
+    gu = torch.randn(4, 2, S, N, N, 2, dtype=torch.float64, device=dev)
+    gnat = empty_prec((lib.qcuMrhsGaugeBytes(geom, prec) // _lib.BYTES[prec],), prec, dev)
+    _lib.check(lib.qcuMrhsPackGauge(gnat.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, stream()))
+    pin = torch.randn(S, 4, N, k, 2, device=dev).to(gnat.dtype)
+    out = torch.empty_like(pin)
+    _lib.check(lib.qcuMrhsDslash(out.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, 0, 0, geom, None,
+                                 stream()))
+    perm = torch.randperm(k, device=dev)
+    pin2 = pin[:, :, :, perm].contiguous()
+    out2 = torch.empty_like(pin2)
+    _lib.check(lib.qcuMrhsDslash(out2.data_ptr(), pin2.data_ptr(), gnat.data_ptr(), prec, k, 0, 0, geom, None,
+                                 stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(out2, out[:, :, :, perm])
+    if prec == _lib.SINGLE:
+        pin3 = (2.0 * pin).contiguous()
+        out3 = torch.empty_like(pin3)
+        _lib.check(lib.qcuMrhsDslash(out3.data_ptr(), pin3.data_ptr(), gnat.data_ptr(), prec, k, 0, 0, geom, None,
+                                     stream()))
+        torch.cuda.synchronize()
+        assert torch.equal(out3, 2.0 * out)  # scaling by 2 is exact in binary floating point
