@@ -43,13 +43,20 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
+    ap.add_argument("--tile", type=str, default=None,
+                    help="fixed tile R,block,lds,tblock[,zblock,hints] for the headline kernel (skips the tuner)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of the headline field size (counter calibration)")
+    ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
+                    help="storage precision of the headline kernel (profiling runs)")
     ap.add_argument("--L", type=int, nargs=4, default=[24, 24, 24, 48])
     ap.add_argument("--nrhs", type=int, default=16)
     return ap.parse_args()
 
 
-def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0):
-    """Tune, then time `steps` applications; returns (ms_per_step, tile, bufs)."""
+def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, tile=None):
+    """Tune (unless a tile is given), then time `steps` applications;
+    returns (ms_per_step, tile)."""
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream()
     sp = ctypes.c_void_p(st.cuda_stream)
@@ -68,20 +75,25 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0):
     gen.manual_seed(seed + 1)
     a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
     b = torch.empty_like(a)
-    best = _lib.Tile()
-    us = ctypes.c_double()
-    _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, ctypes.byref(best),
-                               ctypes.byref(us), sp))
+    if tile is None:
+        best = _lib.Tile()
+        us = ctypes.c_double()
+        _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, ctypes.byref(best),
+                                   ctypes.byref(us), sp))
+        tp = None
+    else:
+        best = _lib.tile(*tile)
+        tp = ctypes.byref(best)
     bufs = [a, b]
     for i in range(warmup):
         _lib.check(lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k,
-                                     i & 1, 0, geom, None, sp))
+                                     i & 1, 0, geom, tp, sp))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for i in range(steps):
         lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k, i & 1, 0,
-                          geom, None, sp)
+                          geom, tp, sp)
     e1.record(st)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
@@ -144,12 +156,25 @@ def main():
     L, N, k = tuple(args.L), 3, args.nrhs
     geom = _lib.geom(L, N)
     S = fields.half_volume(L)
-
-    ms, tile = time_dslash(lib, _lib, torch, geom, _lib.SINGLE, k, args.steps, args.warmup, L, N)
+    if args.copy_calibration:
+        # One device-to-device copy of exactly the headline input field size:
+        # FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters.
+        nbytes = S * k * 4 * N * 8
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+        return
+    pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE}[args.precision]
+    pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8}[args.precision]
+    tile_arg = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
+    ms, tile = time_dslash(lib, _lib, torch, geom, pcode, k, args.steps, args.warmup, L, N, tile=tile_arg)
     value = S * k / (ms * 1e-3)
-    ba = bytes_alg(S, k, N, 4, 4, 4)
+    ba = bytes_alg(S, k, N, pb, pb, pb)
     achieved = ba / (ms * 1e-3) / 1e9
-    wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_fp32"
+    wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_{args.precision}"
     line = {
         "metric": "MRHS dslash sites*RHS/s (SU(3) 24^3x48 nrhs=16 FP32) + %HBM roofline",
         "value": value,
@@ -161,10 +186,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {"fp16": "f16 storage, f32 arithmetic", "bf16": "bf16 storage, f32 arithmetic", "fp32": "f32",
+                  "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded SU(3) links, Gaussian spinors)",
-        "config": {"workload": wl, "lattice": list(L), "nColors": N, "nrhs": k, "storage": "fp32",
-                   "tile": {"rhs_per_lane": tile[0], "block": tile[1], "link_lds": tile[2], "t_block": tile[3]},
+        "config": {"workload": wl, "lattice": list(L), "nColors": N, "nrhs": k, "storage": args.precision,
+                   "tile": dict(zip(("rhs_per_lane", "block", "link_lds", "t_block", "z_block", "cache_hints"), tile)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dslash_c3_fp32_traffic.json"),

@@ -45,7 +45,9 @@ typedef struct qcuMrhsTile_s {
   int rhs_per_lane; /* 1, 2 or 4 right-hand sides per lane */
   int block;        /* lanes per workgroup: 64, 128, 192 or 256 */
   int link_lds;     /* 1: stage each workgroup's links in LDS */
-  int t_block;      /* sites traversed in blocks of t_block time slices (1 = natural order) */
+  int t_block;      /* sites traversed in blocks of t_block time slices ... */
+  int z_block;      /* ... and z_block z slices; (1, 0) = natural order, 0 = Lz */
+  int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);
@@ -80,6 +82,15 @@ int qcuMrhsDslash(void *out, const void *in, const void *gauge, int prec, int nr
 /* out = a * x + b * D in (x packed, same precision and parity as out). */
 int qcuMrhsDslashXpay(void *out, const void *in, const void *x, double a, double b, const void *gauge, int prec,
                       int nrhs, int parity, int dagger, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
+/* General form used by solvers: out = a * x + b * D in with separate storage
+ * precisions for in (and x, and the gauge) and for out, and optional per-RHS
+ * scales (device float arrays of nrhs, NULL = 1): the true input is
+ * stored_in * sc_in[r], the true x is stored_x * sc_x[r], and out is stored
+ * as true_out / sc_out[r]. norm2_out (device double array of nrhs, may be
+ * NULL) receives |true out|^2 per RHS. x may be NULL (then out = b * D in). */
+int qcuMrhsDslashEx(void *out, int out_prec, const void *in, const void *x, int in_prec, double a, double b,
+                    const float *sc_in, const float *sc_x, const float *sc_out, double *norm2_out, const void *gauge,
+                    int nrhs, int parity, int dagger, const qcuMrhsGeom *g, void *stream);
 /* Hopping term on nrhs user spinors (host arrays of device pointers), native
  * gauge in prec; the qcu.h startDslash path without the queue. */
 int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const void *gauge, int prec, int nrhs,

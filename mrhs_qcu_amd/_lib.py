@@ -27,10 +27,10 @@ class Geom(ctypes.Structure):
 
 class Tile(ctypes.Structure):
     _fields_ = [("rhs_per_lane", ctypes.c_int), ("block", ctypes.c_int), ("link_lds", ctypes.c_int),
-                ("t_block", ctypes.c_int)]
+                ("t_block", ctypes.c_int), ("z_block", ctypes.c_int), ("cache_hints", ctypes.c_int)]
 
     def as_tuple(self):
-        return (self.rhs_per_lane, self.block, self.link_lds, self.t_block)
+        return (self.rhs_per_lane, self.block, self.link_lds, self.t_block, self.z_block, self.cache_hints)
 
 
 class CgParam(ctypes.Structure):
@@ -49,7 +49,7 @@ QCU_H_SYMBOLS = ["initGridSize", "pushBackFermions", "startDslash", "loadQcuGaug
                  "fullDslashQcu", "cg_inverter"]
 QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes", "qcuMrhsGaugeBytes",
                       "qcuMrhsPackGauge", "qcuMrhsPackSpinors", "qcuMrhsUnpackSpinors", "qcuMrhsConvert",
-                      "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay",
+                      "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay", "qcuMrhsDslashEx",
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
                       "qcuMrhsBatchDestroy"]
@@ -87,6 +87,7 @@ def _declare(lib):
         "qcuMrhsDslash": (I, [P, P, P, I, I, I, I, G, T, P]),
         "qcuMrhsDslashXpay": (I, [P, P, P, D, D, P, I, I, I, I, G, T, P]),
         "qcuMrhsDslashUser": (I, [PP, PP, P, I, I, I, I, G, P]),
+        "qcuMrhsDslashEx": (I, [P, I, P, P, I, D, D, P, P, P, P, P, I, I, I, G, P]),
         "qcuMrhsDslashReferenceLayout": (I, [PP, PP, P, I, I, I, I, G]),
         "qcuMrhsTune": (I, [P, P, P, I, I, G, T, ctypes.POINTER(D), P]),
         "qcuMrhsGetTile": (I, [G, I, I, T]),
@@ -138,7 +139,7 @@ def ptr_array(ptrs):
     return ctypes.cast(arr, PP), arr
 
 
-def tile(R=1, block=256, lds=0, tblock=1):
+def tile(R=1, block=256, lds=0, tblock=1, zblock=0, hints=0):
     t = Tile()
-    t.rhs_per_lane, t.block, t.link_lds, t.t_block = R, block, lds, tblock
+    t.rhs_per_lane, t.block, t.link_lds, t.t_block, t.z_block, t.cache_hints = R, block, lds, tblock, zblock, hints
     return t

@@ -17,6 +17,8 @@ int prec_bytes(int prec);
 long long sites_per_parity(const Geom &g);
 
 Tile tile_for(const Geom &g, int prec, int k);
+Tile tile_from_c(const qcuMrhsTile &c);
+qcuMrhsTile tile_to_c(const Tile &t);
 void set_tile(const Geom &g, int prec, int k, const Tile &t);
 std::vector<Tile> tile_candidates(const Geom &g, int prec, int k);
 

@@ -115,10 +115,14 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.S = a.subLx * a.Ly * a.Lz * a.Lt;
   for (int d = 0; d < 4; ++d) a.skip[d] = c.geom.grid[d] > 1 ? 1 : 0;
   a.tblock = c.tile.tblock < 1 ? 1 : c.tile.tblock;
+  a.zblock = c.tile.zblock < 1 ? a.Lz : c.tile.zblock;
+  a.nt_store = (c.tile.hints & 1) ? 1 : 0;
+  a.nt_link = (c.tile.hints & 2) ? 1 : 0;
   a.div_subLx = make_fastdiv(uint32_t(a.subLx));
   a.div_row = make_fastdiv(uint32_t(a.subLx * a.Ly));
-  a.div_Lz = make_fastdiv(uint32_t(a.Lz));
   a.div_tb = make_fastdiv(uint32_t(a.tblock));
+  a.div_zb = make_fastdiv(uint32_t(a.zblock));
+  a.div_nzb = make_fastdiv(uint32_t(a.Lz / a.zblock));
   a.div_S = make_fastdiv(uint32_t(a.S));
 }
 
@@ -140,7 +144,8 @@ int launch_dslash(const DslashCall &c, std::string *err)
     return fail(err, "unsupported storage precision"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
-  if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "tblock must divide Lt"), 1;
+  if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
+  if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
   if (!c.gauge) return fail(err, "null gauge"), 1;
 
   DslashArgs a{};
@@ -171,7 +176,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     if (lds) {
       const size_t nsite_max = size_t((B - 1) / a.P + 2);
       shm = nsite_max * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
-      if (shm > 65536) return fail(err, "LDS link staging exceeds 64 KiB for this tile"), 1;
+      if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
     }
     hipError_t e = fn(a, dim3(a.nblocks), dim3(B), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
@@ -200,7 +205,10 @@ int launch_dslash(const DslashCall &c, std::string *err)
     b.P = kk;
     b.div_P = make_fastdiv(uint32_t(kk));
     b.tblock = 1;
+    b.zblock = b.Lz;
     b.div_tb = make_fastdiv(1);
+    b.div_zb = make_fastdiv(uint32_t(b.Lz));
+    b.div_nzb = make_fastdiv(1);
     const unsigned long long lanes = 1ULL * b.S * kk;
     if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
     b.nlanes = uint32_t(lanes);

@@ -50,6 +50,10 @@ struct Reg {
   }
 };
 
+// Dynamic LDS a launch may request: 64 KiB per workgroup minus the kernel's
+// static 2 KiB per-RHS reduction buffer.
+constexpr int kMaxDynLds = 65536 - 2048;
+
 // Geometry of one lattice (local volume + process grid) and colour count.
 struct Geom {
   int L[4];
@@ -62,7 +66,9 @@ struct Tile {
   int R = 1;       // RHS per lane
   int block = 256; // lanes per workgroup
   int lds = 0;     // stage links in LDS
-  int tblock = 1;  // traversal in blocks of tblock time slices
+  int tblock = 1;  // traversal in blocks of tblock time slices ...
+  int zblock = 0;  // ... and zblock z slices (0 = Lz: no z blocking)
+  int hints = 0;   // bit 0: non-temporal output stores, bit 1: non-temporal link loads
 };
 
 struct DslashCall {

@@ -56,10 +56,13 @@ struct DslashArgs {
   int subLx, Ly, Lz, Lt;
   int S;                  // sites per parity
   int skip[4];            // dimension has grid > 1
-  int tblock;             // traversal: blocks of tblock time slices (1 = natural order)
+  int tblock, zblock;     // traversal order (t_out, z_out, t_in, z_in, y, x/2) in blocks of
+                          // tblock time and zblock z slices (natural order: 1, Lz)
+  int nt_store;           // output written with non-temporal stores
+  int nt_link;            // links read with non-temporal loads
   uint32_t nlanes;        // S * P
   uint32_t nblocks;
-  FastDiv div_P, div_subLx, div_row, div_Lz, div_tb, div_S;
+  FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S;
 };
 
 // Hop table: direction D = 2*mu + (backward); projection partner spins a0/a1
@@ -86,26 +89,33 @@ __host__ __device__ constexpr HopEntry hop_entry(int D)
 // Slot of direction D inside a site's 8-link block of the native gauge layout.
 __host__ __device__ constexpr int native_link_slot(int D) { return (D & 1) ? 4 + (D >> 1) : (D >> 1); }
 
-// z * (phase * s)
-template <int KP, class A>
-__device__ __forceinline__ cpx<A> phase_mul(cpx<A> z, A s)
+template <class A>
+__device__ __forceinline__ A fmaA(A a, A b, A c)
 {
-  if constexpr (KP == 0) return cpx<A>{s * z.re, s * z.im};
-  else if constexpr (KP == 1) return cpx<A>{-s * z.re, -s * z.im};
-  else if constexpr (KP == 2) return cpx<A>{-s * z.im, s * z.re};
-  else return cpx<A>{s * z.im, -s * z.re};
+  return __builtin_fmaf(a, b, c);
+}
+template <>
+__device__ __forceinline__ double fmaA<double>(double a, double b, double c)
+{
+  return __builtin_fma(a, b, c);
+}
+
+// base + (phase * s) * z, phase code 0: +1, 1: -1, 2: +i, 3: -i; one FMA per
+// real component (the sign s = +-1 is a run-time operand of the FMA).
+template <int KP, class A>
+__device__ __forceinline__ cpx<A> add_phase(cpx<A> base, cpx<A> z, A s)
+{
+  if constexpr (KP == 0) return cpx<A>{fmaA(s, z.re, base.re), fmaA(s, z.im, base.im)};
+  else if constexpr (KP == 1) return cpx<A>{fmaA(-s, z.re, base.re), fmaA(-s, z.im, base.im)};
+  else if constexpr (KP == 2) return cpx<A>{fmaA(-s, z.im, base.re), fmaA(s, z.re, base.im)};
+  else return cpx<A>{fmaA(s, z.im, base.re), fmaA(-s, z.re, base.im)};
 }
 template <int KP> struct ConjPhase { static constexpr int v = KP < 2 ? KP : (KP == 2 ? 3 : 2); };
 
-template <class A>
-__device__ __forceinline__ void cadd(cpx<A> &a, cpx<A> b)
-{
-  a.re += b.re;
-  a.im += b.im;
-}
-
 // acc += the hop in direction D for one RHS, given the link V and the full
-// neighbour spinor psi[4][N].
+// neighbour spinor psi[4][N]: 2N projections, two N x N complex mat-vecs as
+// FMA chains (one multiply + 2N-1 FMAs per real output) and 4N
+// reconstruction updates.
 template <int N, int D, class A>
 __device__ __forceinline__ void hop_one(cpx<A> (&acc)[4][N], const cpx<A> (&V)[N][N], const cpx<A> (&psi)[4][N], A s)
 {
@@ -113,26 +123,38 @@ __device__ __forceinline__ void hop_one(cpx<A> (&acc)[4][N], const cpx<A> (&V)[N
   cpx<A> h0[N], h1[N];
 #pragma unroll
   for (int c = 0; c < N; ++c) {
-    cpx<A> t0 = phase_mul<e.k0>(psi[e.a0][c], s);
-    cpx<A> t1 = phase_mul<e.k1>(psi[e.a1][c], s);
-    h0[c] = cpx<A>{psi[0][c].re + t0.re, psi[0][c].im + t0.im};
-    h1[c] = cpx<A>{psi[1][c].re + t1.re, psi[1][c].im + t1.im};
+    h0[c] = add_phase<e.k0>(psi[0][c], psi[e.a0][c], s);
+    h1[c] = add_phase<e.k1>(psi[1][c], psi[e.a1][c], s);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    cpx<A> g0 = cz<A>(), g1 = cz<A>();
+    cpx<A> g0, g1;
+    g0.re = V[i][0].re * h0[0].re;
+    g0.im = V[i][0].re * h0[0].im;
+    g1.re = V[i][0].re * h1[0].re;
+    g1.im = V[i][0].re * h1[0].im;
+    g0.re = fmaA(-V[i][0].im, h0[0].im, g0.re);
+    g0.im = fmaA(V[i][0].im, h0[0].re, g0.im);
+    g1.re = fmaA(-V[i][0].im, h1[0].im, g1.re);
+    g1.im = fmaA(V[i][0].im, h1[0].re, g1.im);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
+    for (int j = 1; j < N; ++j) {
       const cpx<A> v = V[i][j];
-      g0.re += v.re * h0[j].re - v.im * h0[j].im;
-      g0.im += v.re * h0[j].im + v.im * h0[j].re;
-      g1.re += v.re * h1[j].re - v.im * h1[j].im;
-      g1.im += v.re * h1[j].im + v.im * h1[j].re;
+      g0.re = fmaA(v.re, h0[j].re, g0.re);
+      g0.im = fmaA(v.re, h0[j].im, g0.im);
+      g1.re = fmaA(v.re, h1[j].re, g1.re);
+      g1.im = fmaA(v.re, h1[j].im, g1.im);
+      g0.re = fmaA(-v.im, h0[j].im, g0.re);
+      g0.im = fmaA(v.im, h0[j].re, g0.im);
+      g1.re = fmaA(-v.im, h1[j].im, g1.re);
+      g1.im = fmaA(v.im, h1[j].re, g1.im);
     }
-    cadd(acc[0][i], g0);
-    cadd(acc[1][i], g1);
-    cadd(acc[e.a0][i], phase_mul<ConjPhase<e.k0>::v>(g0, s));
-    cadd(acc[e.a1][i], phase_mul<ConjPhase<e.k1>::v>(g1, s));
+    acc[0][i].re += g0.re;
+    acc[0][i].im += g0.im;
+    acc[1][i].re += g1.re;
+    acc[1][i].im += g1.im;
+    acc[e.a0][i] = add_phase<ConjPhase<e.k0>::v>(acc[e.a0][i], g0, s);
+    acc[e.a1][i] = add_phase<ConjPhase<e.k1>::v>(acc[e.a1][i], g1, s);
   }
 }
 
@@ -143,25 +165,32 @@ struct SiteGeo {
   int live[8];   // 0 if the term is skipped at a local boundary
 };
 
-__device__ __forceinline__ SiteGeo site_geometry(uint32_t sl, const DslashArgs &a, bool natural)
+// Logical site number -> coordinates for the traversal order
+// (t_out, z_out, t_in, z_in, y, x/2).
+__device__ __forceinline__ void decode_site(uint32_t sl, const DslashArgs &a, uint32_t &xh, uint32_t &y, uint32_t &z,
+                                            uint32_t &t)
+{
+  const uint32_t row = uint32_t(a.subLx) * a.Ly;
+  const uint32_t q = fdiv(sl, a.div_row);
+  const uint32_t r = sl - q * row;
+  y = fdiv(r, a.div_subLx);
+  xh = r - y * a.subLx;
+  const uint32_t q2 = fdiv(q, a.div_zb);
+  const uint32_t zin = q - q2 * a.zblock;
+  const uint32_t q3 = fdiv(q2, a.div_tb);
+  const uint32_t tin = q2 - q3 * a.tblock;
+  const uint32_t tout = fdiv(q3, a.div_nzb);
+  const uint32_t zout = q3 - tout * (a.Lz / a.zblock);
+  z = zout * a.zblock + zin;
+  t = tout * a.tblock + tin;
+}
+
+__device__ __forceinline__ SiteGeo site_geometry(uint32_t sl, const DslashArgs &a)
 {
   SiteGeo g;
   const uint32_t row = uint32_t(a.subLx) * a.Ly;
-  uint32_t q = fdiv(sl, a.div_row);
-  uint32_t r = sl - q * row;
-  uint32_t y = fdiv(r, a.div_subLx);
-  uint32_t xh = r - y * a.subLx;
-  uint32_t z, t;
-  if (natural) {
-    t = fdiv(q, a.div_Lz);
-    z = q - t * a.Lz;
-  } else {
-    uint32_t q2 = fdiv(q, a.div_tb);
-    uint32_t tin = q - q2 * a.tblock;
-    uint32_t tout = fdiv(q2, a.div_Lz);
-    z = q2 - tout * a.Lz;
-    t = tout * a.tblock + tin;
-  }
+  uint32_t xh, y, z, t;
+  decode_site(sl, a, xh, y, z, t);
   const int idx = int(((t * a.Lz + z) * a.Ly + y) * a.subLx + xh);
   g.idx = idx;
   const int eo = int((y + z + t) & 1u);
@@ -263,19 +292,35 @@ __device__ __forceinline__ void do_direction(cpx<A> (&acc)[R][4][N], const Dslas
   for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V, psi[r], s);
 }
 
-// Sites [first, last] of logical order touched by a workgroup.
-__device__ __forceinline__ uint32_t logical_to_phys(uint32_t sl, const DslashArgs &a, bool natural)
+// Checkerboard index of a logical site number.
+__device__ __forceinline__ uint32_t logical_to_phys(uint32_t sl, const DslashArgs &a)
 {
-  if (natural) return sl;
-  const uint32_t row = uint32_t(a.subLx) * a.Ly;
-  uint32_t q = fdiv(sl, a.div_row);
-  uint32_t r = sl - q * row;
-  uint32_t q2 = fdiv(q, a.div_tb);
-  uint32_t tin = q - q2 * a.tblock;
-  uint32_t tout = fdiv(q2, a.div_Lz);
-  uint32_t z = q2 - tout * a.Lz;
-  uint32_t t = tout * a.tblock + tin;
-  return (t * a.Lz + z) * row + r;
+  uint32_t xh, y, z, t;
+  decode_site(sl, a, xh, y, z, t);
+  return ((t * a.Lz + z) * a.Ly + y) * a.subLx + xh;
+}
+
+// Store / load of a small vector with non-temporal hints (16, 8 or 4 byte pieces).
+template <class V>
+__device__ __forceinline__ void store_nt(V *p, const V &v)
+{
+  static_assert(sizeof(V) % 4 == 0, "store size");
+  if constexpr (sizeof(V) % 16 == 0) {
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(&v);
+    u32x4 *dst = reinterpret_cast<u32x4 *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(V) / 16); ++i) __builtin_nontemporal_store(src[i], dst + i);
+  } else if constexpr (sizeof(V) % 8 == 0) {
+    const u32x2 *src = reinterpret_cast<const u32x2 *>(&v);
+    u32x2 *dst = reinterpret_cast<u32x2 *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(V) / 8); ++i) __builtin_nontemporal_store(src[i], dst + i);
+  } else {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&v);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(p);
+#pragma unroll
+    for (int i = 0; i < int(sizeof(V) / 4); ++i) __builtin_nontemporal_store(src[i], dst + i);
+  }
 }
 
 // One kernel for every layout / precision / tile combination.
@@ -290,7 +335,6 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;
-  const bool natural = a.tblock <= 1;
 
   uint32_t sl, slot;
   if constexpr (SITE_MINOR) {
@@ -311,12 +355,13 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
     uint32_t lastl = lane0 + blockDim.x - 1;
     if (lastl >= a.nlanes) lastl = a.nlanes - 1;
     const uint32_t nsite = fdiv(lastl, a.div_P) - first + 1;
-    uint4 *dst = reinterpret_cast<uint4 *>(smem);
-    const uint4 *src = static_cast<const uint4 *>(a.gauge);
+    u32x4 *dst = reinterpret_cast<u32x4 *>(smem);
+    const u32x4 *src = static_cast<const u32x4 *>(a.gauge);
     for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
       const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
-      const uint32_t phys = logical_to_phys(first + sloc, a, natural);
-      dst[i] = src[(size_t(a.parity) * a.S + phys) * chunk16 + w];
+      const uint32_t phys = logical_to_phys(first + sloc, a);
+      const u32x4 *ps = src + (size_t(a.parity) * a.S + phys) * chunk16 + w;
+      dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
     }
     __syncthreads();
     if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * N * N;
@@ -325,7 +370,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
   for (int r = 0; r < R; ++r) nrm[r] = A(0);
   if (active) {
-    const SiteGeo g = site_geometry(sl, a, natural || SITE_MINOR);
+    const SiteGeo g = site_geometry(sl, a);
     if constexpr (GL == GL_NATIVE && !LDS)
       blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * N * N;
 
@@ -381,7 +426,8 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
           v.im *= fo[r];
           o.v[r] = StCvt<CO>::go(v);
         }
-        po[sc * stride] = o;
+        if (a.nt_store) store_nt(po + sc * stride, o);
+        else po[sc * stride] = o;
       }
     } else {
       CO *po;

@@ -25,3 +25,15 @@
 #define QM_PTRS(N, CS, A)                                                             \
   QM_REG(N, 1, ::qmrhs::SL_AOS_PTRS, ::qmrhs::GL_NATIVE, false, false, CS, CS, CS, A) \
   QM_REG(N, 1, ::qmrhs::SL_SOA_PTRS, ::qmrhs::GL_REF_SOA, false, true, CS, CS, CS, A)
+
+// Mixed storage for the solver: sloppy input, FP32 output.
+#define QM_MIXED_TILES(N, CS)                                                                          \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float) \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CS, ::qmrhs::c32, float)  \
+  QM_REG(N, 2, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float) \
+  QM_REG(N, 2, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CS, ::qmrhs::c32, float)
+#define QM_MIXED_R4(N, CS)                                                                             \
+  QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float) \
+  QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CS, ::qmrhs::c32, float)
+#define QM_MIXED_BASIC(N, CS) \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float)

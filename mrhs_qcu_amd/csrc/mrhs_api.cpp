@@ -14,6 +14,7 @@
 
 #include "api_internal.h"
 #include "dslash_dispatch.h"
+#include "solver_blas.h"
 #include "storage_api.h"
 
 namespace qmrhs {
@@ -99,6 +100,12 @@ Tile tile_for(const Geom &g, int prec, int k)
     t.R = 1;
     t.lds = 0;
   }
+  // LDS staging only within the per-workgroup budget.
+  if (t.lds) {
+    const int P = k / t.R;
+    const size_t shm = size_t((t.block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(prec);
+    if (shm > size_t(kMaxDynLds)) t.lds = 0;
+  }
   return t;
 }
 
@@ -116,26 +123,58 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
     for (int lds : {0, 1}) {
       KernelKey key{SL_PACKED, GL_NATIVE, g.N, R, lds, 0, prec, prec, prec};
       if (!find_kernel(key)) continue;
-      for (int block : {64, 128, 256}) {
+      for (int block : {128, 256}) {
         if (lds) {
           const int P = k / R;
           const size_t shm = size_t((block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(prec);
-          if (shm > 65536) continue;
+          if (shm > size_t(kMaxDynLds)) continue;
           if (P < 2) continue;  // no link reuse inside a workgroup
         }
         for (int tb : {1, 2, 4}) {
           if (g.L[3] % tb) continue;
-          Tile t;
-          t.R = R;
-          t.lds = lds;
-          t.block = block;
-          t.tblock = tb;
-          out.push_back(t);
+          for (int zb : {0, 1, 2, 4}) {
+            if (zb > 0 && (g.L[2] % zb || zb >= g.L[2])) continue;
+            if (tb == 1 && zb != 0) continue;  // z blocking only inside time blocks
+            for (int hints : {0, 3}) {
+              Tile t;
+              t.R = R;
+              t.lds = lds;
+              t.block = block;
+              t.tblock = tb;
+              t.zblock = zb;
+              t.hints = hints;
+              out.push_back(t);
+            }
+          }
         }
       }
     }
   }
   return out;
+}
+
+Tile tile_from_c(const qcuMrhsTile &c)
+{
+  Tile t;
+  t.R = c.rhs_per_lane;
+  t.block = c.block;
+  t.lds = c.link_lds;
+  t.tblock = c.t_block;
+  t.zblock = c.z_block;
+  t.hints = c.cache_hints;
+  return t;
+}
+
+qcuMrhsTile tile_to_c(const Tile &t)
+{
+  qcuMrhsTile c;
+  c.rhs_per_lane = t.R;
+  c.block = t.block;
+  c.link_lds = t.lds;
+  c.t_block = t.tblock;
+  c.z_block = t.zblock;
+  c.cache_hints = t.hints;
+  return c;
 }
 
 int run_packed_dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec,
@@ -269,10 +308,7 @@ int qcuMrhsDslash(void *out, const void *in, const void *gauge, int prec, int nr
   Tile t;
   const Tile *tp = nullptr;
   if (tc) {
-    t.R = tc->rhs_per_lane;
-    t.block = tc->block;
-    t.lds = tc->link_lds;
-    t.tblock = tc->t_block;
+    t = tile_from_c(*tc);
     tp = &t;
   }
   if (run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, parity, dagger, g, tp,
@@ -291,15 +327,57 @@ int qcuMrhsDslashXpay(void *out, const void *in, const void *x, double a, double
   Tile t;
   const Tile *tp = nullptr;
   if (tc) {
-    t.R = tc->rhs_per_lane;
-    t.block = tc->block;
-    t.lds = tc->link_lds;
-    t.tblock = tc->t_block;
+    t = tile_from_c(*tc);
     tp = &t;
   }
   if (run_packed_dslash(out, in, x, a, b, gauge, prec, nrhs, parity, dagger, g, tp, static_cast<hipStream_t>(stream),
                         &err))
     return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsDslashEx(void *out, int out_prec, const void *in, const void *x, int in_prec, double a, double b,
+                    const float *sc_in, const float *sc_x, const float *sc_out, double *norm2_out, const void *gauge,
+                    int nrhs, int parity, int dagger, const qcuMrhsGeom *gc, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DslashCall c;
+  c.sl = SL_PACKED;
+  c.gl = GL_NATIVE;
+  c.prec_s = c.prec_g = in_prec;
+  c.prec_o = out_prec;
+  c.geom = g;
+  c.tile = tile_for(g, in_prec, nrhs);
+  c.k = nrhs;
+  c.parity = parity;
+  c.dagger = dagger;
+  c.out = out;
+  c.in = in;
+  c.x = x;
+  c.xa = a;
+  c.xb = b;
+  c.gauge = gauge;
+  c.sc_in = sc_in;
+  c.sc_x = sc_x;
+  c.sc_out = sc_out;
+  c.stream = st;
+  double *partial = nullptr;
+  const long long nb = dslash_num_blocks(c);
+  if (norm2_out) {
+    if (hipMalloc(&partial, size_t(nb) * nrhs * sizeof(double)) != hipSuccess) return api_fail("allocation failed");
+    c.partial = partial;
+  }
+  int rc = launch_dslash(c, &err);
+  if (!rc && norm2_out) {
+    hipError_t e = blas_reduce(partial, nb, nrhs, norm2_out, st);
+    if (e != hipSuccess) rc = 1, err = hipGetErrorString(e);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = 1, err = "synchronisation failed";
+  }
+  if (partial) (void)hipFree(partial);
+  if (rc) return api_fail(err);
   return 0;
 }
 
@@ -333,11 +411,7 @@ int qcuMrhsGetTile(const qcuMrhsGeom *gc, int prec, int nrhs, qcuMrhsTile *tile)
   std::string err;
   if (!geom_from_c(gc, &g, &err)) return api_fail(err);
   if (!tile) return api_fail("null tile");
-  Tile t = tile_for(g, prec, nrhs);
-  tile->rhs_per_lane = t.R;
-  tile->block = t.block;
-  tile->link_lds = t.lds;
-  tile->t_block = t.tblock;
+  *tile = tile_to_c(tile_for(g, prec, nrhs));
   return 0;
 }
 
@@ -378,12 +452,7 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   set_tile(g, prec, nrhs, best_tile);
-  if (best) {
-    best->rhs_per_lane = best_tile.R;
-    best->block = best_tile.block;
-    best->link_lds = best_tile.lds;
-    best->t_block = best_tile.tblock;
-  }
+  if (best) *best = tile_to_c(best_tile);
   if (best_us) *best_us = best_t;
   return 0;
 }

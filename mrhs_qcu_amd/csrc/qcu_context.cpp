@@ -355,6 +355,11 @@ void shiftGaugeStorage(void *dst, void *src, ShiftDirection dir, int Lx, int Ly,
 // C ABI (reference src/qcu.cu:125-181).
 static qcu::Qcu *qcu_ptr = nullptr;
 
+namespace qmrhs {
+// Colour count of the current context (3 when none was created).
+int qcu_context_colors() { return qcu_ptr ? qcu_ptr->geom().N : 3; }
+}  // namespace qmrhs
+
 extern "C" {
 
 void initGridSize(QcuGrid_t *grid, QcuParam *p_param, int nColors)
@@ -414,8 +419,9 @@ int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void
   QcuLattDesc<4> latt(g->L[0], g->L[1], g->L[2], g->L[3]);
   QcuProcDesc<4> proc(g->grid[0] > 0 ? g->grid[0] : 1, g->grid[1] > 0 ? g->grid[1] : 1,
                       g->grid[2] > 0 ? g->grid[2] : 1, g->grid[3] > 0 ? g->grid[3] : 1);
-  static std::unique_ptr<qcu::ExecutionStreams> streams;
-  if (!streams) streams.reset(new qcu::ExecutionStreams());
+  // Created once and kept for the life of the process (never destroyed at
+  // exit, when the HIP runtime may already be gone).
+  static qcu::ExecutionStreams *streams = new qcu::ExecutionStreams();
   qcu::instantiateDslash(latt, proc, parity, dagger, 0.0, const_cast<void **>(in_soa), const_cast<void **>(out_soa),
                          gauge_soa, DSLASH_WILSON, static_cast<QCU_PRECISION>(prec), g->nColors, nrhs, *streams);
   return 0;
@@ -423,8 +429,7 @@ int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void
 
 }  // extern "C"
 
-__attribute__((destructor)) static void destroyQcu()
-{
-  delete qcu_ptr;
-  qcu_ptr = nullptr;
-}
+// The reference releases its context from a library destructor
+// (src/qcu.cu:176-181). Here the context is released by the next initGridSize
+// only: at process exit the HIP runtime may already be torn down, and device
+// memory is returned with the process anyway.

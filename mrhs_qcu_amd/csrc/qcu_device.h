@@ -102,6 +102,10 @@ template <> struct StCvt<cb16> {
   }
 };
 
+// Native vector types for the non-temporal load/store builtins.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 // R consecutive stored complex numbers, loaded/stored as one vector access.
 template <class C, int R>
 struct alignas(sizeof(C) * R <= 16 ? sizeof(C) * R : 16) CVec {

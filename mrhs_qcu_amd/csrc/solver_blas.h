@@ -1,0 +1,34 @@
+// Vector kernels and device scalar state of the multi-RHS CG (host entry
+// points; all asynchronous on `st`).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qmrhs {
+
+constexpr int kBlasBlock = 256;
+constexpr int kBlasMaxRhs = 256;
+
+// Per-RHS scalars, all device arrays of length k.
+struct CgState {
+  double *rr, *rr_new, *pap, *alpha, *beta, *pnorm2, *pnorm, *b2;
+  int *done, *sloppy_conv;
+  float *sc_p, *sc_t, *sc_q, *sc_t2;  // sloppy storage scales (true = stored * sc)
+};
+
+// Workgroups of every vector kernel for n complex numbers and k RHS (the
+// partial-sum buffers hold blas_num_blocks(n, k) * k doubles).
+long long blas_num_blocks(long long n, int k);
+
+hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const void *p, const double *alpha,
+                          double *partial, long long n, int k, hipStream_t st);
+hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
+                         const float *sc_p, double *partial, long long n, int k, hipStream_t st);
+hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
+hipError_t blas_residual(void *r, const void *b, const void *Ax, double *partial, long long n, int k, hipStream_t st);
+hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st);
+hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
+hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st);
+hipError_t cg_alpha(const CgState &s, int k, hipStream_t st);
+hipError_t cg_beta(const CgState &s, int k, double stop2, hipStream_t st);
+
+}  // namespace qmrhs

@@ -1,0 +1,356 @@
+// Vector kernels of the multi-RHS CG on packed fields [S][4][N][k].
+//
+// Every kernel is grid-stride with a thread count that is a multiple of k, so
+// each lane keeps one right-hand side for its whole walk and accumulates its
+// per-RHS sums in a register; a workgroup folds them into LDS and writes one
+// partial per RHS: [nblocks][k], summed by cg_reduce_kernel. No global atomics,
+// so every reduction is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "qcu_device.h"
+#include "solver_blas.h"
+
+namespace qmrhs {
+
+namespace {
+
+template <class C> struct Acc;
+template <> struct Acc<c64> { using T = double; };
+template <> struct Acc<c32> { using T = float; };
+template <> struct Acc<c16> { using T = float; };
+template <> struct Acc<cb16> { using T = float; };
+
+__device__ __forceinline__ void block_partial(double v, int rhs, int k, double *partial)
+{
+  __shared__ double red[kBlasMaxRhs];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+  atomicAdd(&red[rhs], v);
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) partial[size_t(blockIdx.x) * k + i] = red[i];
+}
+
+// r -= alpha Ap ; x += alpha p ; partial |r|^2
+template <class CK>
+__global__ void __launch_bounds__(kBlasBlock) update_rx_kernel(CK *__restrict__ r, CK *__restrict__ x,
+                                                               const CK *__restrict__ Ap, const CK *__restrict__ p,
+                                                               const double *__restrict__ alpha, double *partial,
+                                                               long long n, int k)
+{
+  using A = typename Acc<CK>::T;
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = int(g % k);
+  const A a = A(alpha[rhs]);
+  double acc = 0.0;
+  for (long long e = g; e < n; e += T) {
+    cpx<A> rr = ld_cvt<A>(r[e]), ap = ld_cvt<A>(Ap[e]), xx = ld_cvt<A>(x[e]), pp = ld_cvt<A>(p[e]);
+    rr.re -= a * ap.re;
+    rr.im -= a * ap.im;
+    xx.re += a * pp.re;
+    xx.im += a * pp.im;
+    r[e] = StCvt<CK>::go(rr);
+    x[e] = StCvt<CK>::go(xx);
+    acc += double(rr.re) * rr.re + double(rr.im) * rr.im;
+  }
+  block_partial(acc, rhs, k, partial);
+}
+
+// p = r + beta p ; p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2
+template <class CK, class CS>
+__global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p, CS *__restrict__ ps,
+                                                              const CK *__restrict__ r,
+                                                              const double *__restrict__ beta,
+                                                              const float *__restrict__ sc_p, double *partial,
+                                                              long long n, int k)
+{
+  using A = typename Acc<CK>::T;
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = int(g % k);
+  const A b = A(beta[rhs]);
+  const float inv = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
+  double acc = 0.0;
+  for (long long e = g; e < n; e += T) {
+    cpx<A> pp = ld_cvt<A>(p[e]), rr = ld_cvt<A>(r[e]);
+    pp.re = rr.re + b * pp.re;
+    pp.im = rr.im + b * pp.im;
+    p[e] = StCvt<CK>::go(pp);
+    if (ps) ps[e] = StCvt<CS>::go(cpx<float>{float(pp.re) * inv, float(pp.im) * inv});
+    acc += double(pp.re) * pp.re + double(pp.im) * pp.im;
+  }
+  block_partial(acc, rhs, k, partial);
+}
+
+// partial |v|^2
+template <class C>
+__global__ void __launch_bounds__(kBlasBlock) norm2_kernel(const C *__restrict__ v, double *partial, long long n,
+                                                           int k)
+{
+  using A = typename Acc<C>::T;
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = int(g % k);
+  double acc = 0.0;
+  for (long long e = g; e < n; e += T) {
+    cpx<A> z = ld_cvt<A>(v[e]);
+    acc += double(z.re) * z.re + double(z.im) * z.im;
+  }
+  block_partial(acc, rhs, k, partial);
+}
+
+// r = b - Ax (FP64) ; partial |r|^2
+__global__ void __launch_bounds__(kBlasBlock) residual_kernel(c64 *__restrict__ r, const c64 *__restrict__ b,
+                                                              const c64 *__restrict__ Ax, double *partial,
+                                                              long long n, int k)
+{
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = int(g % k);
+  double acc = 0.0;
+  for (long long e = g; e < n; e += T) {
+    c64 bb = b[e], aa = Ax[e];
+    c64 rr{bb.x - aa.x, bb.y - aa.y};
+    r[e] = rr;
+    acc += rr.x * rr.x + rr.y * rr.y;
+  }
+  block_partial(acc, rhs, k, partial);
+}
+
+// xh += x (x converted to FP64), then x = 0
+template <class CK>
+__global__ void __launch_bounds__(kBlasBlock) accumulate_zero_kernel(c64 *__restrict__ xh, CK *__restrict__ x,
+                                                                     long long n)
+{
+  const long long T = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += T) {
+    cpx<double> v = ld_cvt<double>(x[e]);
+    c64 h = xh[e];
+    h.x += v.re;
+    h.y += v.im;
+    xh[e] = h;
+    x[e] = StCvt<CK>::go(cpx<double>{0.0, 0.0});
+  }
+}
+
+// Fill with zeros (a kernel, so the solver loop stays capturable in a graph).
+template <class C>
+__global__ void __launch_bounds__(kBlasBlock) zero_kernel(C *__restrict__ v, long long n)
+{
+  const long long T = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += T)
+    v[e] = StCvt<C>::go(cpx<float>{0.f, 0.f});
+}
+
+// out[r] = sum_b partial[b][r] for r < k, one workgroup per RHS.
+__global__ void __launch_bounds__(256) reduce_kernel(const double *__restrict__ partial, long long nb, int k,
+                                                     double *__restrict__ out)
+{
+  __shared__ double red[256];
+  const int r = blockIdx.x;
+  double acc = 0.0;
+  for (long long b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[b * k + r];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = red[0];
+}
+
+// alpha = rr / pAp for live RHS; pnorm = sqrt(sum |p|^2).
+__global__ void alpha_kernel(CgState s, int k)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double pap = s.pap[r];
+  s.alpha[r] = (s.done[r] || !(pap > 0.0)) ? 0.0 : s.rr[r] / pap;
+  s.pnorm[r] = sqrt(s.pnorm2[r]);
+}
+
+// beta = rr_new / rr; rr = rr_new; next sloppy scales for p and the
+// intermediate fields (bounds that keep binary16 away from overflow).
+__global__ void beta_kernel(CgState s, int k, double stop2)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double rn = s.rr_new[r];
+  const bool live = !s.done[r];
+  const double beta = (live && s.rr[r] > 0.0) ? rn / s.rr[r] : 0.0;
+  s.beta[r] = beta;
+  if (live) s.rr[r] = rn;
+  s.sloppy_conv[r] = (rn <= stop2 * s.b2[r]) ? 1 : 0;
+  const double bound = sqrt(rn) + fabs(beta) * s.pnorm[r];
+  float sp = float(bound / 1024.0);
+  if (!(sp > 1e-30f)) sp = 1e-30f;
+  s.sc_p[r] = sp;
+  s.sc_t[r] = 16.0f * sp;
+  s.sc_q[r] = 4.0f * sp;
+  s.sc_t2[r] = 64.0f * sp;
+}
+
+// rr = |b^|^2, beta = 0, done only for a zero right-hand side, scales for p = r.
+__global__ void init_state_kernel(CgState s, int k)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double b2 = s.b2[r];
+  s.rr[r] = b2;
+  s.rr_new[r] = b2;
+  s.beta[r] = 0.0;
+  s.alpha[r] = 0.0;
+  s.pnorm[r] = 0.0;
+  s.pnorm2[r] = 0.0;
+  s.done[r] = b2 > 0.0 ? 0 : 1;
+  s.sloppy_conv[r] = s.done[r];
+  float sp = float(sqrt(b2) / 1024.0);
+  if (!(sp > 1e-30f)) sp = 1e-30f;
+  s.sc_p[r] = sp;
+  s.sc_t[r] = 16.0f * sp;
+  s.sc_q[r] = 4.0f * sp;
+  s.sc_t2[r] = 64.0f * sp;
+}
+
+// After a reliable update: restart the recurrence from the true residual and
+// mark the right-hand sides that meet the tolerance.
+__global__ void check_true_kernel(CgState s, const double *rr_true, int k, double stop2)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double rt = rr_true[r];
+  if (!s.done[r]) s.rr[r] = rt;
+  if (rt <= stop2 * s.b2[r]) s.done[r] = 1;
+}
+
+// Workgroup count: a multiple of q (the least q with q * kBlasBlock % k == 0,
+// so the thread count is a multiple of k), about 8 workgroups per CU.
+unsigned grid_for(long long n, int k)
+{
+  long long q = 1;
+  while ((long long)kBlasBlock * q % k) ++q;
+  long long nb = (2048 + q - 1) / q * q;
+  const long long need = (n + kBlasBlock - 1) / kBlasBlock;
+  if (nb > need) nb = (need + q - 1) / q * q;
+  if (nb < q) nb = q;
+  return unsigned(nb);
+}
+
+}  // namespace
+
+long long blas_num_blocks(long long n, int k) { return grid_for(n, k); }
+
+#define QM_GRID(n, k) dim3(grid_for(n, k)), dim3(kBlasBlock), 0, st
+
+hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const void *p, const double *alpha,
+                          double *partial, long long n, int k, hipStream_t st)
+{
+  if (prec_k == PREC_DOUBLE)
+    hipLaunchKernelGGL(update_rx_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<c64 *>(x),
+                       static_cast<const c64 *>(Ap), static_cast<const c64 *>(p), alpha, partial, n, k);
+  else
+    hipLaunchKernelGGL(update_rx_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(r), static_cast<c32 *>(x),
+                       static_cast<const c32 *>(Ap), static_cast<const c32 *>(p), alpha, partial, n, k);
+  return hipGetLastError();
+}
+
+hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
+                         const float *sc_p, double *partial, long long n, int k, hipStream_t st)
+{
+  if (prec_k == PREC_DOUBLE) {
+    hipLaunchKernelGGL((update_p_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(p), static_cast<c64 *>(ps),
+                       static_cast<const c64 *>(r), beta, sc_p, partial, n, k);
+  } else {
+    switch (prec_s) {
+      case PREC_HALF:
+        hipLaunchKernelGGL((update_p_kernel<c32, c16>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c16 *>(ps),
+                           static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+        break;
+      case PREC_BF16:
+        hipLaunchKernelGGL((update_p_kernel<c32, cb16>), QM_GRID(n, k), static_cast<c32 *>(p),
+                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+        break;
+      default:
+        hipLaunchKernelGGL((update_p_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c32 *>(ps),
+                           static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st)
+{
+  switch (prec) {
+    case PREC_DOUBLE:
+      hipLaunchKernelGGL(norm2_kernel<c64>, QM_GRID(n, k), static_cast<const c64 *>(v), partial, n, k);
+      break;
+    case PREC_SINGLE:
+      hipLaunchKernelGGL(norm2_kernel<c32>, QM_GRID(n, k), static_cast<const c32 *>(v), partial, n, k);
+      break;
+    case PREC_HALF:
+      hipLaunchKernelGGL(norm2_kernel<c16>, QM_GRID(n, k), static_cast<const c16 *>(v), partial, n, k);
+      break;
+    default:
+      hipLaunchKernelGGL(norm2_kernel<cb16>, QM_GRID(n, k), static_cast<const cb16 *>(v), partial, n, k);
+  }
+  return hipGetLastError();
+}
+
+hipError_t blas_residual(void *r, const void *b, const void *Ax, double *partial, long long n, int k, hipStream_t st)
+{
+  hipLaunchKernelGGL(residual_kernel, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
+                     static_cast<const c64 *>(Ax), partial, n, k);
+  return hipGetLastError();
+}
+
+hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st)
+{
+  if (prec_k == PREC_DOUBLE)
+    hipLaunchKernelGGL(accumulate_zero_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c64 *>(x), n);
+  else
+    hipLaunchKernelGGL(accumulate_zero_kernel<c32>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c32 *>(x), n);
+  return hipGetLastError();
+}
+
+hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st)
+{
+  switch (prec) {
+    case PREC_DOUBLE: hipLaunchKernelGGL(zero_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(v), n); break;
+    case PREC_SINGLE: hipLaunchKernelGGL(zero_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(v), n); break;
+    case PREC_HALF: hipLaunchKernelGGL(zero_kernel<c16>, QM_GRID(n, k), static_cast<c16 *>(v), n); break;
+    default: hipLaunchKernelGGL(zero_kernel<cb16>, QM_GRID(n, k), static_cast<cb16 *>(v), n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(reduce_kernel, dim3(k), dim3(256), 0, st, partial, nb, k, out);
+  return hipGetLastError();
+}
+
+hipError_t cg_init_state(const CgState &s, int k, hipStream_t st)
+{
+  hipLaunchKernelGGL(init_state_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, k);
+  return hipGetLastError();
+}
+
+hipError_t cg_check_true(const CgState &s, const double *rr_true, int k, double stop2, hipStream_t st)
+{
+  hipLaunchKernelGGL(check_true_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, rr_true, k, stop2);
+  return hipGetLastError();
+}
+
+hipError_t cg_alpha(const CgState &s, int k, hipStream_t st)
+{
+  hipLaunchKernelGGL(alpha_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, k);
+  return hipGetLastError();
+}
+
+hipError_t cg_beta(const CgState &s, int k, double stop2, hipStream_t st)
+{
+  hipLaunchKernelGGL(beta_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, k, stop2);
+  return hipGetLastError();
+}
+
+}  // namespace qmrhs

@@ -1,0 +1,441 @@
+// Multi-RHS even-odd preconditioned CG on the normal equations.
+//
+// M^ = 1 - kappa^2 D_eo D_oe on the even checkerboard (D_eo: output parity 0,
+// D_oe: output parity 1, D the hopping term of the dslash kernels); M^+ is the
+// same with the dagger flag set, because D(s=-1) = g5 D(s=+1) g5. The solver
+// returns x with M^+ M^ x = M^+ b for every right-hand side.
+//
+// Precision scheme (sloppy_prec != DOUBLE):
+//   * dslash storage in sloppy_prec (FP16 / BF16 / FP32) with FP32 arithmetic;
+//     binary16 fields carry a per-RHS scale (true = stored * scale) chosen
+//     from the CG scalars so no value can overflow;
+//   * Krylov vectors r, p, x_k and M^+M^ p in FP32;
+//   * all dot products and norms accumulated in FP64 (per-workgroup partials,
+//     deterministic reduction);
+//   * every reliable_interval iterations (and whenever the iterated residual
+//     says converged) x is folded into the FP64 solution and the residual is
+//     recomputed in FP64 from the FP64 operator.
+// With sloppy_prec == DOUBLE every field is FP64 and the true residual is
+// recomputed only when the iterated residual reaches the tolerance.
+// All scalars live on the device; the host reads k flags per check.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+#include "dslash_dispatch.h"
+#include "qcu.h"
+#include "qcu_mrhs.h"
+#include "solver_blas.h"
+#include "storage_api.h"
+
+namespace qmrhs {
+
+int qcu_context_colors();
+hipError_t cg_init_state(const CgState &s, int k, hipStream_t st);
+hipError_t cg_check_true(const CgState &s, const double *rr_true, int k, double stop2, hipStream_t st);
+
+namespace {
+
+struct DevMem {
+  std::vector<void *> ptrs;
+  ~DevMem()
+  {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  T *alloc(size_t count, std::string *err, bool *ok)
+  {
+    void *p = nullptr;
+    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+      *ok = false;
+      if (err && err->empty()) *err = "device allocation failed in the solver";
+      return nullptr;
+    }
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+  void *bytes(size_t n, std::string *err, bool *ok) { return alloc<unsigned char>(n, err, ok); }
+};
+
+struct Solver {
+  Geom g;
+  int k;
+  double kappa;
+  hipStream_t st;
+  long long S, n;  // sites per parity, complex numbers per field
+  double *partial = nullptr;
+  size_t partial_cap = 0;  // doubles
+  std::string *err;
+
+  bool dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec_in,
+              int prec_out, int parity, int dagger, const float *sc_in, const float *sc_x, const float *sc_out,
+              bool norm)
+  {
+    DslashCall c;
+    c.sl = SL_PACKED;
+    c.gl = GL_NATIVE;
+    c.prec_s = prec_in;
+    c.prec_g = prec_in;
+    c.prec_o = prec_out;
+    c.geom = g;
+    c.tile = tile_for(g, prec_in, k);
+    c.k = k;
+    c.parity = parity;
+    c.dagger = dagger;
+    c.out = out;
+    c.in = in;
+    c.x = x;
+    c.xa = xa;
+    c.xb = xb;
+    c.gauge = gauge;
+    c.sc_in = sc_in;
+    c.sc_x = sc_x;
+    c.sc_out = sc_out;
+    c.stream = st;
+    if (norm) {
+      if (size_t(dslash_num_blocks(c)) * k > partial_cap) {
+        *err = "partial buffer too small";
+        return false;
+      }
+      c.partial = partial;
+    }
+    if (launch_dslash(c, err)) return false;
+    if (norm && hipGetLastError() != hipSuccess) return false;
+    return true;
+  }
+
+  long long dslash_blocks(int prec) const
+  {
+    DslashCall c;
+    c.sl = SL_PACKED;
+    c.geom = g;
+    c.tile = tile_for(g, prec, k);
+    c.k = k;
+    return dslash_num_blocks(c);
+  }
+
+  // out = M^ in (dagger 0) or M^+ in (dagger 1), FP64, using tmp (odd parity).
+  bool apply_m64(void *out, const void *in, void *tmp, const void *gauge, int dagger)
+  {
+    return dslash(tmp, in, nullptr, 0.0, 1.0, gauge, PREC_DOUBLE, PREC_DOUBLE, 1, dagger, nullptr, nullptr, nullptr,
+                  false) &&
+           dslash(out, tmp, in, 1.0, -kappa * kappa, gauge, PREC_DOUBLE, PREC_DOUBLE, 0, dagger, nullptr, nullptr,
+                  nullptr, false);
+  }
+};
+
+bool hip_ok(hipError_t e, std::string *err, const char *what)
+{
+  if (e == hipSuccess) return true;
+  *err = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+}  // namespace
+
+int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom &g, qcuMrhsCgParam *prm,
+             double *true_res, hipStream_t st, std::string *err)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  if (k < 1 || k > kBlasMaxRhs) return *err = "nrhs must be in 1..256", 1;
+  const int ps = prm->sloppy_prec;
+  if (ps != PREC_DOUBLE && ps != PREC_SINGLE && ps != PREC_HALF && ps != PREC_BF16)
+    return *err = "unsupported sloppy precision", 1;
+  if (!(prm->tol > 0.0)) return *err = "tolerance must be positive", 1;
+  const bool mixed = ps != PREC_DOUBLE;
+  const int pk = mixed ? PREC_SINGLE : PREC_DOUBLE;  // Krylov precision
+  Solver s;
+  s.g = g;
+  s.k = k;
+  s.kappa = prm->kappa;
+  s.st = st;
+  s.err = err;
+  s.S = sites_per_parity(g);
+  s.n = s.S * 4 * g.N * k;
+  const long long n = s.n;
+  const int interval = prm->reliable_interval > 0 ? prm->reliable_interval : 50;
+  const int max_iter = prm->max_iter > 0 ? prm->max_iter : 10000;
+  const double stop2 = prm->tol * prm->tol;
+
+  DevMem mem;
+  bool ok = true;
+  const size_t b64 = size_t(n) * 16, bk = size_t(n) * prec_bytes(pk), bs = size_t(n) * prec_bytes(ps);
+  void *bhat = mem.bytes(b64, err, &ok);
+  void *rh = mem.bytes(b64, err, &ok);
+  void *t64 = mem.bytes(b64, err, &ok);
+  void *ax64 = mem.bytes(b64, err, &ok);
+  void *r = mem.bytes(bk, err, &ok);
+  void *p = mem.bytes(bk, err, &ok);
+  void *ap = mem.bytes(bk, err, &ok);
+  void *xk = mixed ? mem.bytes(bk, err, &ok) : x_out;
+  void *p_s = mixed ? mem.bytes(bs, err, &ok) : p;
+  void *t_s = mixed ? mem.bytes(bs, err, &ok) : t64;
+  void *q_s = mixed ? mem.bytes(bs, err, &ok) : ax64;
+  void *t2_s = mixed ? mem.bytes(bs, err, &ok) : t64;
+  void *gs = mixed ? mem.bytes(size_t(s.S) * 2 * 8 * g.N * g.N * prec_bytes(ps), err, &ok) : const_cast<void *>(gauge64);
+  const long long nb_blas = blas_num_blocks(n, k);
+  long long nb_max = nb_blas;
+  nb_max = std::max(nb_max, s.dslash_blocks(ps));
+  nb_max = std::max(nb_max, s.dslash_blocks(PREC_DOUBLE));
+  s.partial_cap = size_t(nb_max) * k;
+  s.partial = mem.alloc<double>(s.partial_cap, err, &ok);
+  double *partial_p = mem.alloc<double>(size_t(nb_blas) * k, err, &ok);
+  CgState cs;
+  cs.rr = mem.alloc<double>(k, err, &ok);
+  cs.rr_new = mem.alloc<double>(k, err, &ok);
+  cs.pap = mem.alloc<double>(k, err, &ok);
+  cs.alpha = mem.alloc<double>(k, err, &ok);
+  cs.beta = mem.alloc<double>(k, err, &ok);
+  cs.pnorm2 = mem.alloc<double>(k, err, &ok);
+  cs.pnorm = mem.alloc<double>(k, err, &ok);
+  cs.b2 = mem.alloc<double>(k, err, &ok);
+  cs.done = mem.alloc<int>(k, err, &ok);
+  cs.sloppy_conv = mem.alloc<int>(k, err, &ok);
+  cs.sc_p = mem.alloc<float>(k, err, &ok);
+  cs.sc_t = mem.alloc<float>(k, err, &ok);
+  cs.sc_q = mem.alloc<float>(k, err, &ok);
+  cs.sc_t2 = mem.alloc<float>(k, err, &ok);
+  double *rr_true = mem.alloc<double>(k, err, &ok);
+  if (!ok) return 1;
+  if (mixed) {
+    std::string e2;
+    if (convert_complex(gs, ps, gauge64, PREC_DOUBLE, s.S * 2 * 8 * g.N * g.N, st, &e2)) return *err = e2, 1;
+  }
+  const bool scaled = mixed && ps == PREC_HALF;
+  const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
+  const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
+
+#define QM_TRY(expr)                                     \
+  do {                                                   \
+    if (!(expr)) return (err->empty() ? (*err = "solver step failed: " #expr) : *err), 1; \
+  } while (0)
+#define QM_HIP(expr) QM_TRY(hip_ok((expr), err, #expr))
+
+  // b^ = M^+ b ; |b^|^2
+  QM_TRY(s.apply_m64(bhat, b, t64, gauge64, 1));
+  QM_HIP(blas_norm2(PREC_DOUBLE, bhat, s.partial, n, k, st));
+  QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.b2, st));
+  QM_HIP(cg_init_state(cs, k, st));  // rr = b2, beta = 0, done = 0, scales from |b^|
+  // x = 0, r = b^, p = r
+  QM_HIP(blas_zero(PREC_DOUBLE, x_out, n, k, st));
+  if (mixed) QM_HIP(blas_zero(pk, xk, n, k, st));
+  if (mixed) {
+    std::string e2;
+    if (convert_complex(r, pk, bhat, PREC_DOUBLE, n, st, &e2)) return *err = e2, 1;
+  } else {
+    QM_HIP(hipMemcpyAsync(r, bhat, b64, hipMemcpyDeviceToDevice, st));
+  }
+  QM_HIP(blas_zero(pk, p, n, k, st));
+  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, sc_p, partial_p, n, k, st));
+
+  std::vector<int> done(k, 0);
+  std::vector<double> rrt(k, 0.0), b2h(k, 0.0), rr_ru(k, 0.0);
+  int iter = 0, updates = 0, since_update = 0;
+  bool converged = false;
+  const double kk2 = -prm->kappa * prm->kappa;
+  // Reliable-update trigger: the iterated residual of some live RHS fell by
+  // delta since that RHS's last true residual (the sloppy and the true
+  // residual then cannot drift apart), or every live RHS reached the
+  // tolerance, or reliable_interval iterations passed.
+  const double delta = ps == PREC_SINGLE ? 1e-2 : 1e-1;
+  const double delta2 = mixed ? delta * delta : 0.0;
+  // QCU_MRHS_CG_VERBOSE=1 prints the iterated and true residuals.
+  const char *verb_env = getenv("QCU_MRHS_CG_VERBOSE");
+  const bool verbose = verb_env && verb_env[0] == '1';
+
+  // Per-iteration flags come back through pinned host memory, read one
+  // iteration late so the device queue never drains while the host decides.
+  struct Pinned {
+    void *p = nullptr;
+    ~Pinned()
+    {
+      if (p) (void)hipHostFree(p);
+    }
+  } pin;
+  QM_HIP(hipHostMalloc(&pin.p, 2 * size_t(k) * (sizeof(double) + sizeof(int)), hipHostMallocDefault));
+  double *h_rr[2] = {static_cast<double *>(pin.p), static_cast<double *>(pin.p) + k};
+  int *h_conv[2] = {reinterpret_cast<int *>(h_rr[1] + k), reinterpret_cast<int *>(h_rr[1] + k) + k};
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events()
+    {
+      for (auto &x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  QM_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
+  QM_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+
+  // True residual: x_h += x_k (mixed), r_h = b^ - M^+M^ x_h, rr_true, done
+  // flags; the Krylov residual restarts from r_h.
+  auto reliable_update = [&]() -> bool {
+    if (mixed && !hip_ok(blas_accumulate_zero(pk, x_out, xk, n, k, st), err, "accumulate")) return false;
+    if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
+    if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
+    if (!hip_ok(blas_residual(rh, bhat, ax64, s.partial, n, k, st), err, "residual")) return false;
+    if (!hip_ok(blas_reduce(s.partial, nb_blas, k, rr_true, st), err, "reduce")) return false;
+    if (!hip_ok(cg_check_true(cs, rr_true, k, stop2, st), err, "check")) return false;
+    if (mixed) {
+      std::string e2;
+      if (convert_complex(r, pk, rh, PREC_DOUBLE, n, st, &e2)) return *err = e2, false;
+    } else if (!hip_ok(hipMemcpyAsync(r, rh, b64, hipMemcpyDeviceToDevice, st), err, "copy")) {
+      return false;
+    }
+    ++updates;
+    since_update = 0;
+    if (!hip_ok(hipMemcpyAsync(done.data(), cs.done, k * sizeof(int), hipMemcpyDeviceToHost, st), err, "d2h"))
+      return false;
+    if (!hip_ok(hipMemcpyAsync(rrt.data(), rr_true, k * sizeof(double), hipMemcpyDeviceToHost, st), err, "d2h"))
+      return false;
+    if (!hip_ok(hipStreamSynchronize(st), err, "sync")) return false;
+    converged = true;
+    double mx = 0;
+    for (int i = 0; i < k; ++i) {
+      converged = converged && done[i];
+      rr_ru[i] = rrt[i];
+      mx = std::max(mx, b2h[i] > 0 ? std::sqrt(rrt[i] / b2h[i]) : 0.0);
+    }
+    if (verbose) fprintf(stderr, "cg iter %d: reliable update %d, true relative residual max %.3e\n", iter, updates, mx);
+    return true;
+  };
+
+  QM_HIP(hipMemcpyAsync(b2h.data(), cs.b2, k * sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(done.data(), cs.done, k * sizeof(int), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  rr_ru = b2h;
+  converged = true;
+  for (int i = 0; i < k; ++i) converged = converged && done[i];
+
+  bool pending = false;
+  while (!converged && iter < max_iter) {
+    const int cur = iter & 1;
+    // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
+    QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
+    QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
+    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.pap, st));
+    QM_HIP(blas_reduce(partial_p, nb_blas, k, cs.pnorm2, st));
+    QM_HIP(cg_alpha(cs, k, st));
+    // Ap = q - kappa^2 D^+_eo D^+_oe q
+    QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
+    QM_TRY(s.dslash(ap, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, false));
+    // r -= alpha Ap ; x += alpha p ; beta ; p = r + beta p
+    QM_HIP(blas_update_rx(pk, r, xk, ap, p, cs.alpha, s.partial, n, k, st));
+    QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.rr_new, st));
+    QM_HIP(cg_beta(cs, k, stop2, st));
+    QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, sc_p, partial_p, n, k, st));
+    QM_HIP(hipMemcpyAsync(h_rr[cur], cs.rr, k * sizeof(double), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(h_conv[cur], cs.sloppy_conv, k * sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipEventRecord(ev.e[cur], st));
+    ++iter;
+    ++since_update;
+    if (!pending) {
+      pending = true;
+      continue;
+    }
+    // Decide on the previous iteration's flags while this one runs.
+    const int prv = cur ^ 1;
+    QM_HIP(hipEventSynchronize(ev.e[prv]));
+    bool all_conv = true, drop = false;
+    double mx = 0;
+    for (int i = 0; i < k; ++i) {
+      if (done[i]) continue;
+      all_conv = all_conv && h_conv[prv][i];
+      drop = drop || (h_rr[prv][i] < delta2 * rr_ru[i]);
+      mx = std::max(mx, b2h[i] > 0 ? std::sqrt(h_rr[prv][i] / b2h[i]) : 0.0);
+    }
+    if (verbose && iter % 10 == 0) fprintf(stderr, "cg iter %d: iterated relative residual max %.3e\n", iter - 1, mx);
+    if (all_conv || drop || (mixed && since_update >= interval) || iter == max_iter) {
+      QM_TRY(reliable_update());
+      pending = false;  // flags queued before the update are stale
+    }
+  }
+  if (!converged && since_update > 0) QM_TRY(reliable_update());
+  QM_HIP(hipMemcpyAsync(rrt.data(), rr_true, k * sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipMemcpyAsync(b2h.data(), cs.b2, k * sizeof(double), hipMemcpyDeviceToHost, st));
+  QM_HIP(hipStreamSynchronize(st));
+  if (true_res)
+    for (int i = 0; i < k; ++i) true_res[i] = b2h[i] > 0 ? std::sqrt(rrt[i] / b2h[i]) : 0.0;
+  prm->iterations = iter;
+  prm->reliable_updates = updates;
+  prm->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+#undef QM_TRY
+#undef QM_HIP
+  return 0;
+}
+
+}  // namespace qmrhs
+
+using namespace qmrhs;
+
+extern "C" {
+
+// qcu.h cg_inverter (a stub in the reference, src/qcu.cu:173): solves the full
+// Wilson system (1 - kappa D) x = b for one right-hand side. x and b are full
+// spinors in the user layout, even checkerboard [S][4][N] followed by the odd
+// one; gauge is the user AoS gauge [4][2][S][N][N]; all complex<double>.
+// Schur complement on the even sites:
+//   (1 - kappa^2 D_eo D_oe) x_e = b_e + kappa D_eo b_o,  x_o = b_o + kappa D_oe x_e,
+// the first solved by the mixed-precision CG above to relative residual
+// p_max_prec of its normal equations.
+void cg_inverter(void *x_vector, void *b_vector, void *gauge, QcuParam *param, double p_max_prec, double p_kappa)
+{
+  auto die = [](const std::string &m) {
+    fprintf(stderr, "cg_inverter: %s\n", m.c_str());
+    exit(1);
+  };
+  if (!x_vector || !b_vector || !gauge || !param) die("null argument");
+  Geom g;
+  for (int d = 0; d < 4; ++d) {
+    g.L[d] = param->lattice_size[d];
+    g.grid[d] = 1;
+    if (g.L[d] <= 0 || (g.L[d] & 1)) die("lattice sizes must be positive and even");
+  }
+  g.N = qcu_context_colors();
+  const long long S = sites_per_parity(g);
+  const size_t half = size_t(S) * 4 * g.N * 16;
+  std::string err;
+  void *gn = nullptr, *bh = nullptr;
+  if (hipMalloc(&gn, size_t(S) * 2 * 8 * g.N * g.N * 16) != hipSuccess || hipMalloc(&bh, half) != hipSuccess)
+    die("device allocation failed");
+  hipStream_t st = nullptr;
+  char *x = static_cast<char *>(x_vector), *b = static_cast<char *>(b_vector);
+  if (pack_gauge(gn, PREC_DOUBLE, gauge, PREC_DOUBLE, g.N, g.L, st, &err)) die(err);
+  // b^_e = b_e + kappa D_eo b_o
+  if (run_packed_dslash(bh, b + half, b, 1.0, p_kappa, gn, PREC_DOUBLE, 1, 0, 0, g, nullptr, st, &err)) die(err);
+  qcuMrhsCgParam prm{};
+  prm.kappa = p_kappa;
+  prm.tol = p_max_prec;
+  prm.max_iter = 100000;
+  prm.reliable_interval = 50;
+  prm.sloppy_prec = PREC_HALF;
+  double res = 0.0;
+  if (cg_solve(x, bh, gn, 1, g, &prm, &res, st, &err)) die(err);
+  // x_o = b_o + kappa D_oe x_e
+  if (run_packed_dslash(x + half, x, b + half, 1.0, p_kappa, gn, PREC_DOUBLE, 1, 1, 0, g, nullptr, st, &err))
+    die(err);
+  if (hipStreamSynchronize(st) != hipSuccess) die("synchronisation failed");
+  (void)hipFree(gn);
+  (void)hipFree(bh);
+  printf("cg_inverter: %d iterations, %d reliable updates, true relative residual %.3e, %.3f s\n", prm.iterations,
+         prm.reliable_updates, res, prm.seconds);
+}
+
+int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, const qcuMrhsGeom *gc, qcuMrhsCgParam *p,
+                   double *true_res, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!x || !b || !gauge_f64 || !p) return api_fail("null argument");
+  if (cg_solve(x, b, gauge_f64, nrhs, g, p, true_res, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  return 0;
+}
+
+}  // extern "C"
