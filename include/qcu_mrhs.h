@@ -47,7 +47,9 @@ typedef struct qcuMrhsTile_s {
   int link_lds;     /* 1: stage each workgroup's links in LDS */
   int t_block;      /* sites traversed in blocks of t_block time slices ... */
   int z_block;      /* ... and z_block z slices; (1, 0) = natural order, 0 = Lz */
-  int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads */
+  int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads;
+                       bit 2: keep the hardware workgroup order (no XCD-contiguous remap) */
+  int occupancy;    /* cap on resident workgroups per CU (3..8), 0 = no cap */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);

@@ -118,6 +118,7 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.zblock = c.tile.zblock < 1 ? a.Lz : c.tile.zblock;
   a.nt_store = (c.tile.hints & 1) ? 1 : 0;
   a.nt_link = (c.tile.hints & 2) ? 1 : 0;
+  a.xcd_map = (c.tile.hints & 4) ? 0 : 1;
   a.div_subLx = make_fastdiv(uint32_t(a.subLx));
   a.div_row = make_fastdiv(uint32_t(a.subLx * a.Ly));
   a.div_tb = make_fastdiv(uint32_t(a.tblock));
@@ -177,6 +178,12 @@ int launch_dslash(const DslashCall &c, std::string *err)
       const size_t nsite_max = size_t((B - 1) / a.P + 2);
       shm = nsite_max * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
       if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
+    }
+    if (c.tile.occ > 0) {
+      // Pad the LDS request so at most `occ` workgroups fit on a CU.
+      const size_t cap = size_t(kLdsPerCu / c.tile.occ) - 2048 - 512;
+      if (cap > size_t(kMaxDynLds)) return fail(err, "occupancy cap below 3 workgroups per CU is not reachable"), 1;
+      if (shm < cap) shm = cap;
     }
     hipError_t e = fn(a, dim3(a.nblocks), dim3(B), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;

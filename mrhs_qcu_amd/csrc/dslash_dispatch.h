@@ -53,6 +53,8 @@ struct Reg {
 // Dynamic LDS a launch may request: 64 KiB per workgroup minus the kernel's
 // static 2 KiB per-RHS reduction buffer.
 constexpr int kMaxDynLds = 65536 - 2048;
+// LDS of one gfx950 compute unit.
+constexpr int kLdsPerCu = 160 * 1024;
 
 // Geometry of one lattice (local volume + process grid) and colour count.
 struct Geom {
@@ -68,7 +70,9 @@ struct Tile {
   int lds = 0;     // stage links in LDS
   int tblock = 1;  // traversal in blocks of tblock time slices ...
   int zblock = 0;  // ... and zblock z slices (0 = Lz: no z blocking)
-  int hints = 0;   // bit 0: non-temporal output stores, bit 1: non-temporal link loads
+  int hints = 0;   // bit 0: non-temporal output stores, bit 1: non-temporal link loads,
+                   // bit 2: keep the hardware workgroup order (no XCD remap)
+  int occ = 0;     // cap on resident workgroups per CU (LDS padding); 0 = no cap
 };
 
 struct DslashCall {

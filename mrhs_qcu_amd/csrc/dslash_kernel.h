@@ -60,6 +60,7 @@ struct DslashArgs {
                           // tblock time and zblock z slices (natural order: 1, Lz)
   int nt_store;           // output written with non-temporal stores
   int nt_link;            // links read with non-temporal loads
+  int xcd_map;            // remap workgroups so each XCD owns a contiguous range
   uint32_t nlanes;        // S * P
   uint32_t nblocks;
   FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S;
@@ -278,18 +279,34 @@ __device__ __forceinline__ void load_link_refsoa(cpx<A> (&V)[N][N], const Dslash
     }
 }
 
-template <int D, int N, int R, int SL, int GL, bool LDS, class CS, class CG, class A>
-__device__ __forceinline__ void do_direction(cpx<A> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
-                                             const CG *blk8, A s)
+// Link of direction D for the output site (native layout: from the site's
+// 8-link block, in global memory or staged in LDS; reference layout: gathered).
+template <int D, int N, int GL, class CG, class A>
+__device__ __forceinline__ void fetch_link(cpx<A> (&V)[N][N], const DslashArgs &a, const SiteGeo &g, const CG *blk8)
 {
-  if (!g.live[D]) return;
-  cpx<A> V[N][N];
   if constexpr (GL == GL_NATIVE) load_link_native<N, CG, A>(V, blk8, native_link_slot(D));
   else load_link_refsoa<N, CG, A>(V, a, D, g.idx, g.nbr[D]);
-  cpx<A> psi[R][4][N];
-  load_spinor<N, R, SL, CS, A>(psi, a, a.in, g.nbr[D], slot);
+}
+
+// Software-pipelined sweep over the 8 directions: the neighbour spinor (and
+// link) of direction D+1 are requested before the arithmetic of direction D,
+// so each wave keeps one direction of loads in flight while it computes.
+// Every neighbour index is in range (periodic wrap), so the prefetch is
+// unconditional; a skipped boundary term only skips the arithmetic.
+template <int D, int N, int R, int SL, int GL, class CS, class CG, class A>
+__device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
+                                      const CG *blk8, A s, cpx<A> (&psi_cur)[R][4][N], cpx<A> (&V_cur)[N][N],
+                                      cpx<A> (&psi_nxt)[R][4][N], cpx<A> (&V_nxt)[N][N])
+{
+  if constexpr (D + 1 < 8) {
+    load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[D + 1], slot);
+    fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
+  }
+  if (g.live[D]) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V, psi[r], s);
+    for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V_cur, psi_cur[r], s);
+  }
+  if constexpr (D + 1 < 8) sweep<D + 1, N, R, SL, GL, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
 }
 
 // Checkerboard index of a logical site number.
@@ -331,7 +348,7 @@ template <int N, int R, int SL, int GL, bool LDS, bool SITE_MINOR, class CS, cla
 __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t blk = xcd_remap(blockIdx.x, a.nblocks);
+  const uint32_t blk = a.xcd_map ? xcd_remap(blockIdx.x, a.nblocks) : blockIdx.x;
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;
@@ -383,14 +400,12 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cz<A>();
 
-    do_direction<0, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<1, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<2, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<3, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<4, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<5, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<6, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
-    do_direction<7, N, R, SL, GL, LDS, CS, CG, A>(acc, a, g, slot, blk8, s);
+    {
+      cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
+      load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
+      fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
+      sweep<0, N, R, SL, GL, CS, CG, A>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
+    }
 
     // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.
     A fin[R], fx[R], fo[R];

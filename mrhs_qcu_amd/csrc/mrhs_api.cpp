@@ -162,6 +162,7 @@ Tile tile_from_c(const qcuMrhsTile &c)
   t.tblock = c.t_block;
   t.zblock = c.z_block;
   t.hints = c.cache_hints;
+  t.occ = c.occupancy;
   return t;
 }
 
@@ -174,6 +175,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.t_block = t.tblock;
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
+  c.occupancy = t.occ;
   return c;
 }
 
