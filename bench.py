@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+_GAUGE_CACHE = {}
 
 
 def bytes_alg(S, k, N, b_in, b_out, b_u):
@@ -42,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
+    ap.add_argument("--cg-only", action="store_true", help="only the CG comparison (printed as one JSON line)")
+    ap.add_argument("--cg-L", type=int, nargs=4, default=[32, 32, 32, 64])
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
     ap.add_argument("--tile", type=str, default=None,
                     help="fixed tile R,block,lds,tblock[,zblock,hints] for the headline kernel (skips the tuner)")
@@ -64,13 +67,14 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     import numpy as np
 
     S = fields.half_volume(L)
-    g = fields.gauge_field(L, N, seed=seed)
-    gu = torch.from_numpy(g.view(np.float64).reshape(-1)).to(dev)
-    del g
+    key = (tuple(L), N, seed)
+    if key not in _GAUGE_CACHE:
+        _GAUGE_CACHE.clear()
+        _GAUGE_CACHE[key] = torch.view_as_real(fields.gauge_field_device(L, N, seed=seed, device=dev)).reshape(-1)
+    gu = _GAUGE_CACHE[key]
     dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
     gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
     _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
-    del gu
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
     a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
@@ -98,6 +102,63 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     return ms, best.as_tuple()
+
+
+def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0):
+    """Even-odd preconditioned MRHS CG on M^+M^ to a true relative residual
+    tol: pure FP64 vs mixed precision (FP16 dslash storage, FP32 Krylov
+    vectors, FP64 reductions and reliable updates), same GPU, same inputs.
+    Sources: RHS 0..min(k,4N)-1 unit deltas (spin-colour component r) at
+    seeded even sites, the rest Gaussian noise."""
+    import numpy as np
+
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    geom = _lib.geom(L, N)
+    S = fields.half_volume(L)
+    gu = torch.view_as_real(fields.gauge_field_device(L, N, seed=seed, device=dev)).reshape(-1)
+    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, _lib.DOUBLE) // 8, dtype=torch.float64, device=dev)
+    _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), _lib.DOUBLE, gu.data_ptr(), _lib.DOUBLE, geom, sp))
+    del gu
+    b = torch.zeros(S, 4, N, k, 2, dtype=torch.float64, device=dev)
+    npt = min(k, 4 * N)
+    rng = np.random.Generator(np.random.PCG64(seed + 11))
+    sites = rng.integers(0, S, size=npt)
+    for r in range(npt):
+        b[int(sites[r]), (r % (4 * N)) // N, r % N, r, 0] = 1.0
+    if k > npt:
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed + 12)
+        b[:, :, :, npt:, :] = torch.randn(S, 4, N, k - npt, 2, generator=gen, device=dev, dtype=torch.float64)
+    out = {"workload": f"su{N}_eo_cg_mdagm_{'x'.join(map(str, L))}_nrhs{k}_kappa{kappa}", "tol": tol,
+           "sources": f"{npt} point sources + {k - npt} Gaussian"}
+    sols = {}
+    for name, sloppy in (("fp64", _lib.DOUBLE), ("mixed_fp16", _lib.HALF)):
+        x = torch.empty_like(b)
+        prm = _lib.CgParam()
+        res = (ctypes.c_double * k)()
+        # Untimed warm-up: tunes the operator tiles and allocates the solver's
+        # work vectors (both kept for the timed solve).
+        prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 2, 50, sloppy
+        _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp))
+        prm = _lib.CgParam()
+        prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 20000, 50, sloppy
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[name] = {"seconds": wall, "iterations": prm.iterations, "reliable_updates": prm.reliable_updates,
+                     "max_true_rel_residual": max(res), "converged": bool(max(res) <= tol)}
+        sols[name] = x
+    d = (sols["mixed_fp16"] - sols["fp64"]).permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
+    n = sols["fp64"].permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
+    out["max_rel_solution_diff"] = float((d / n).max())
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_fp16"]["seconds"]
+    return out
 
 
 def cpu_baseline(L, N, k, budget_s=12.0):
@@ -167,6 +228,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
         return
+    if args.cg_only:
+        print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))), flush=True)
+        return
     pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE}[args.precision]
     pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8}[args.precision]
     tile_arg = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
@@ -212,6 +276,8 @@ def main():
             "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tile": t3}
         line["extra"] = extra
+        if not args.no_cg:
+            line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))
     if not args.no_cpu and not args.profile:
         line["cpu_baseline"] = cpu_baseline(L, N, k)
     print(json.dumps(line), flush=True)

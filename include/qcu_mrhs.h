@@ -120,6 +120,8 @@ typedef struct qcuMrhsCgParam_s {
   int max_iter;          /* iteration cap */
   int reliable_interval; /* mixed precision: iterations between FP64 residual updates */
   int sloppy_prec;       /* QCU_MRHS_DOUBLE: FP64 CG; HALF / BF16 / SINGLE: mixed precision */
+  double reliable_delta; /* mixed precision: true residual recomputed once the iterated residual of a
+                            RHS fell by this factor since its last update (0 = precision default) */
   /* outputs */
   int iterations;
   int reliable_updates;
@@ -132,6 +134,10 @@ typedef struct qcuMrhsCgParam_s {
  * residual of each RHS recomputed in FP64. Synchronous. */
 int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, const qcuMrhsGeom *g, qcuMrhsCgParam *p,
                    double *true_res, void *stream);
+/* The solver keeps its work vectors between calls (no allocation when the
+ * shape repeats) and tunes its operator tiles on first use; this releases
+ * the work vectors. */
+int qcuMrhsSolverRelease(void);
 
 /* ---- batching ------------------------------------------------------------ */
 typedef struct qcuMrhsProblem_s {

@@ -38,7 +38,7 @@ class Tile(ctypes.Structure):
 class CgParam(ctypes.Structure):
     _fields_ = [("kappa", ctypes.c_double), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("reliable_interval", ctypes.c_int), ("sloppy_prec", ctypes.c_int),
-                ("iterations", ctypes.c_int), ("reliable_updates", ctypes.c_int), ("seconds", ctypes.c_double)]
+                ("reliable_delta", ctypes.c_double), ("iterations", ctypes.c_int), ("reliable_updates", ctypes.c_int), ("seconds", ctypes.c_double)]
 
 
 class Problem(ctypes.Structure):
@@ -53,7 +53,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsPackGauge", "qcuMrhsPackSpinors", "qcuMrhsUnpackSpinors", "qcuMrhsConvert",
                       "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay", "qcuMrhsDslashEx",
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
-                      "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
+                      "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
                       "qcuMrhsBatchDestroy"]
 
 _lib = None
@@ -95,6 +95,7 @@ def _declare(lib):
         "qcuMrhsGetTile": (I, [G, I, I, T]),
         "qcuMrhsKernelCount": (I, []),
         "qcuMrhsCgSolve": (I, [P, P, P, I, G, ctypes.POINTER(CgParam), ctypes.POINTER(D), P]),
+        "qcuMrhsSolverRelease": (I, []),
         "qcuMrhsBatchCreate": (I, [ctypes.POINTER(Problem), I, ctypes.POINTER(P), P]),
         "qcuMrhsBatchRun": (I, [P, P]),
         "qcuMrhsBatchDestroy": (I, [P]),

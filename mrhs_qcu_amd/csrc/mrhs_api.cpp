@@ -109,6 +109,12 @@ Tile tile_for(const Geom &g, int prec, int k)
   return t;
 }
 
+bool tuned(const Geom &g, int prec, int k)
+{
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  return tune_cache().count(tune_key(g, prec, k)) > 0;
+}
+
 void set_tile(const Geom &g, int prec, int k, const Tile &t)
 {
   std::lock_guard<std::mutex> lk(tune_mutex());

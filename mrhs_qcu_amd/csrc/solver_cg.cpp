@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdlib>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -43,22 +44,59 @@ hipError_t cg_check_true(const CgState &s, const double *rr_true, int k, double 
 
 namespace {
 
+// Device work vectors kept between solves: a block is handed back to the
+// pool at the end of a solve and reused by the next request of the same size.
+struct Pool {
+  std::vector<std::pair<size_t, void *>> free_blocks;
+  std::mutex m;
+  void *take(size_t bytes)
+  {
+    std::lock_guard<std::mutex> lk(m);
+    for (size_t i = 0; i < free_blocks.size(); ++i)
+      if (free_blocks[i].first == bytes) {
+        void *p = free_blocks[i].second;
+        free_blocks.erase(free_blocks.begin() + long(i));
+        return p;
+      }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+  }
+  void give(size_t bytes, void *p)
+  {
+    std::lock_guard<std::mutex> lk(m);
+    free_blocks.emplace_back(bytes, p);
+  }
+  void release()
+  {
+    std::lock_guard<std::mutex> lk(m);
+    for (auto &b : free_blocks) (void)hipFree(b.second);
+    free_blocks.clear();
+  }
+};
+Pool &pool()
+{
+  static Pool *p = new Pool();  // never destroyed: process exit may outlive the HIP runtime
+  return *p;
+}
+
 struct DevMem {
-  std::vector<void *> ptrs;
+  std::vector<std::pair<size_t, void *>> blocks;
   ~DevMem()
   {
-    for (void *p : ptrs) (void)hipFree(p);
+    for (auto &b : blocks) pool().give(b.first, b.second);
   }
   template <class T>
   T *alloc(size_t count, std::string *err, bool *ok)
   {
-    void *p = nullptr;
-    if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+    const size_t bytes = count * sizeof(T);
+    void *p = pool().take(bytes);
+    if (!p) {
       *ok = false;
       if (err && err->empty()) *err = "device allocation failed in the solver";
       return nullptr;
     }
-    ptrs.push_back(p);
+    blocks.emplace_back(bytes, p);
     return static_cast<T *>(p);
   }
   void *bytes(size_t n, std::string *err, bool *ok) { return alloc<unsigned char>(n, err, ok); }
@@ -140,6 +178,31 @@ bool hip_ok(hipError_t e, std::string *err, const char *what)
 
 }  // namespace
 
+bool tuned(const Geom &g, int prec, int k);
+
+static bool tune_now(void *out, void *in, const void *gauge, int prec, int k, const Geom &g, hipStream_t st,
+                     std::string *err)
+{
+  // Any finite input works for timing; zero it so no stale bits propagate.
+  if (hipMemsetAsync(in, 0, size_t(sites_per_parity(g)) * 4 * g.N * k * prec_bytes(prec), st) != hipSuccess) {
+    *err = "memset failed";
+    return false;
+  }
+  qcuMrhsGeom gc;
+  for (int d = 0; d < 4; ++d) {
+    gc.L[d] = g.L[d];
+    gc.grid[d] = g.grid[d];
+  }
+  gc.nColors = g.N;
+  qcuMrhsTile best;
+  double us = 0;
+  if (qcuMrhsTune(out, in, gauge, prec, k, &gc, &best, &us, st) != 0) {
+    *err = std::string("tuning failed: ") + qcuMrhsLastError();
+    return false;
+  }
+  return true;
+}
+
 int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom &g, qcuMrhsCgParam *prm,
              double *true_res, hipStream_t st, std::string *err)
 {
@@ -181,9 +244,9 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   void *t2_s = mixed ? mem.bytes(bs, err, &ok) : t64;
   void *gs = mixed ? mem.bytes(size_t(s.S) * 2 * 8 * g.N * g.N * prec_bytes(ps), err, &ok) : const_cast<void *>(gauge64);
   const long long nb_blas = blas_num_blocks(n, k);
-  long long nb_max = nb_blas;
-  nb_max = std::max(nb_max, s.dslash_blocks(ps));
-  nb_max = std::max(nb_max, s.dslash_blocks(PREC_DOUBLE));
+  // Partial sums for the largest workgroup count any tile can use
+  // (one RHS per lane, 64-lane workgroups).
+  const long long nb_max = std::max(nb_blas, (s.S * k + 63) / 64);
   s.partial_cap = size_t(nb_max) * k;
   s.partial = mem.alloc<double>(s.partial_cap, err, &ok);
   double *partial_p = mem.alloc<double>(size_t(nb_blas) * k, err, &ok);
@@ -208,6 +271,10 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     std::string e2;
     if (convert_complex(gs, ps, gauge64, PREC_DOUBLE, s.S * 2 * 8 * g.N * g.N, st, &e2)) return *err = e2, 1;
   }
+  // Tune the operator tiles for this shape once per process (the tuner
+  // overwrites the scratch fields it times on; they are initialised below).
+  if (!tuned(g, ps, k) && !tune_now(t_s, p_s, gs, ps, k, g, st, err)) return 1;
+  if (mixed && !tuned(g, PREC_DOUBLE, k) && !tune_now(t64, ax64, gauge64, PREC_DOUBLE, k, g, st, err)) return 1;
   const bool scaled = mixed && ps == PREC_HALF;
   const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
   const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
@@ -244,7 +311,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // delta since that RHS's last true residual (the sloppy and the true
   // residual then cannot drift apart), or every live RHS reached the
   // tolerance, or reliable_interval iterations passed.
-  const double delta = ps == PREC_SINGLE ? 1e-2 : 1e-1;
+  const double delta = prm->reliable_delta > 0.0 ? prm->reliable_delta
+                       : (ps == PREC_SINGLE ? 1e-3 : (ps == PREC_HALF ? 1e-2 : 1e-1));
   const double delta2 = mixed ? delta * delta : 0.0;
   // QCU_MRHS_CG_VERBOSE=1 prints the iterated and true residuals.
   const char *verb_env = getenv("QCU_MRHS_CG_VERBOSE");
@@ -425,6 +493,12 @@ void cg_inverter(void *x_vector, void *b_vector, void *gauge, QcuParam *param, d
   (void)hipFree(bh);
   printf("cg_inverter: %d iterations, %d reliable updates, true relative residual %.3e, %.3f s\n", prm.iterations,
          prm.reliable_updates, res, prm.seconds);
+}
+
+int qcuMrhsSolverRelease(void)
+{
+  pool().release();
+  return 0;
 }
 
 int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, const qcuMrhsGeom *gc, qcuMrhsCgParam *p,

@@ -143,3 +143,31 @@ def to_packed(psi):
 def from_packed(p):
     """packed [S][4][N][k] -> [k][S][4][N]."""
     return np.ascontiguousarray(np.transpose(p, (3, 0, 1, 2)))
+
+
+def gauge_field_device(L, N, seed=0, device="cuda:0", antiperiodic_t=True, dtype=None):
+    """The same construction as gauge_field (complex Gaussian, modified
+    Gram-Schmidt on rows, det fixed to 1, antiperiodic time folded in), done
+    with torch on a device for the large benchmark lattices. Returns a
+    complex128 tensor [4][2][S][N][N]; the random stream is torch's, so the
+    field differs from gauge_field's numpy stream with the same seed."""
+    import torch
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    S = half_volume(L)
+    shape = (4 * 2 * S, N, N)
+    m = torch.complex(torch.randn(shape, generator=gen, device=device, dtype=torch.float64),
+                      torch.randn(shape, generator=gen, device=device, dtype=torch.float64))
+    for i in range(N):
+        for j in range(i):
+            proj = torch.sum(torch.conj(m[:, j, :]) * m[:, i, :], dim=-1, keepdim=True)
+            m[:, i, :] -= proj * m[:, j, :]
+        m[:, i, :] /= torch.linalg.vector_norm(m[:, i, :], dim=-1, keepdim=True)
+    det = torch.linalg.det(m)
+    m[:, N - 1, :] *= torch.conj(det)[:, None]
+    m = m.reshape(4, 2, S, N, N)
+    if antiperiodic_t:
+        t = torch.as_tensor(site_coords(L)[3], device=device)
+        m[3, :, t == L[3] - 1] *= -1.0
+    return m
