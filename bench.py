@@ -240,7 +240,7 @@ def main():
     achieved = ba / (ms * 1e-3) / 1e9
     wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_{args.precision}"
     line = {
-        "metric": "MRHS dslash sites*RHS/s (SU(3) 24^3x48 nrhs=16 FP32) + %HBM roofline",
+        "metric": "MRHS dslash sites\u00b7RHS/s + %HBM roofline; mixed vs FP64 CG time to 1e-10",
         "value": value,
         "unit": "sites*RHS/s",
         "n_gpus": 1,
@@ -257,7 +257,7 @@ def main():
                    "tile": dict(zip(("rhs_per_lane", "block", "link_lds", "t_block", "z_block", "cache_hints"), tile)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("dslash_c3_fp32_traffic.json"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(f"{wl}_traffic.json"),
                      "bytes_alg_per_launch": ba},
     }
     if not args.profile and not args.no_extra:

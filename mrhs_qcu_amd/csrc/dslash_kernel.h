@@ -19,6 +19,8 @@
 // neighbour spinor blocks are read as contiguous RHS-innermost vectors.
 #pragma once
 
+#include <type_traits>
+
 #include "qcu_device.h"
 
 namespace qmrhs {
@@ -309,6 +311,133 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
   if constexpr (D + 1 < 8) sweep<D + 1, N, R, SL, GL, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
 }
 
+// ---------------------------------------------------------------------------
+// binary16 storage path. Spinor and link components stay packed (re, im)
+// half pairs and every complex multiply-accumulate of the mat-vec is two
+// v_dot2_f32_f16 (d = s0.lo*s1.lo + s0.hi*s1.hi + s2, products and sum in
+// FP32):
+//   Im(V h) = dot2(V, hs),  Re(V h) = dot2(Vc, hs),
+// with hs = (Im h, Re h) and Vc = (-Im V, Re V). The projection produces hs
+// directly (one packed-half FMA per colour, the half swap folded into its
+// operand selection), and Vc is formed once per link and shared by the R
+// right-hand sides of the lane.
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2 as_h2(const c16 &v)
+{
+  h2 r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+__device__ __forceinline__ h2 swap_h2(h2 h) { return __builtin_shufflevector(h, h, 1, 0); }
+
+// swap(base + (phase * s) * z) for the phase codes of add_phase; ss = (s, s).
+template <int KP>
+__device__ __forceinline__ h2 proj_swapped_h2(h2 base, h2 z, h2 ss)
+{
+  if constexpr (KP == 0) return __builtin_elementwise_fma(ss, swap_h2(z), swap_h2(base));
+  else if constexpr (KP == 1) return __builtin_elementwise_fma(-ss, swap_h2(z), swap_h2(base));
+  else if constexpr (KP == 2) {  // i s z = s (-Im z, Re z); swapped: (s Re z, -s Im z)
+    const h2 sg = {ss.x, _Float16(-ss.y)};
+    return __builtin_elementwise_fma(sg, z, swap_h2(base));
+  } else {  // -i s z = s (Im z, -Re z); swapped: (-s Re z, s Im z)
+    const h2 sg = {_Float16(-ss.x), ss.y};
+    return __builtin_elementwise_fma(sg, z, swap_h2(base));
+  }
+}
+
+// Vc = (-Im V, Re V).
+__device__ __forceinline__ h2 link_c_h2(h2 v)
+{
+  const h2 m = {_Float16(-1.0f), _Float16(1.0f)};
+  return swap_h2(v) * m;
+}
+
+// acc += hop in direction D for one RHS. Each mat-vec chain starts from -z
+// (z an SGPR zero): the negated scalar addend selects the three-address form
+// of the first v_dot2_f32_f16, so no register has to be zeroed or copied.
+template <int N, int D>
+__device__ __forceinline__ void hop_one_h2(cpx<float> (&acc)[4][N], const h2 (&V)[N][N], const h2 (&Vc)[N][N],
+                                           const h2 (&psi)[4][N], h2 ss, float s, float z)
+{
+  constexpr HopEntry e = hop_entry(D);
+  h2 h0s[N], h1s[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    h0s[c] = proj_swapped_h2<e.k0>(psi[0][c], psi[e.a0][c], ss);
+    h1s[c] = proj_swapped_h2<e.k1>(psi[1][c], psi[e.a1][c], ss);
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    cpx<float> g0, g1;
+    g0.re = __builtin_amdgcn_fdot2(Vc[i][0], h0s[0], -z, false);
+    g0.im = __builtin_amdgcn_fdot2(V[i][0], h0s[0], -z, false);
+    g1.re = __builtin_amdgcn_fdot2(Vc[i][0], h1s[0], -z, false);
+    g1.im = __builtin_amdgcn_fdot2(V[i][0], h1s[0], -z, false);
+#pragma unroll
+    for (int j = 1; j < N; ++j) {
+      g0.re = __builtin_amdgcn_fdot2(Vc[i][j], h0s[j], g0.re, false);
+      g0.im = __builtin_amdgcn_fdot2(V[i][j], h0s[j], g0.im, false);
+      g1.re = __builtin_amdgcn_fdot2(Vc[i][j], h1s[j], g1.re, false);
+      g1.im = __builtin_amdgcn_fdot2(V[i][j], h1s[j], g1.im, false);
+    }
+    acc[0][i].re += g0.re;
+    acc[0][i].im += g0.im;
+    acc[1][i].re += g1.re;
+    acc[1][i].im += g1.im;
+    acc[e.a0][i] = add_phase<ConjPhase<e.k0>::v>(acc[e.a0][i], g0, s);
+    acc[e.a1][i] = add_phase<ConjPhase<e.k1>::v>(acc[e.a1][i], g1, s);
+  }
+}
+
+template <int N, int R, class A>
+__device__ __forceinline__ void load_spinor_h2(h2 (&psi)[R][4][N], const DslashArgs &a, int site, int slot)
+{
+  const CVec<c16, R> *p = reinterpret_cast<const CVec<c16, R> *>(static_cast<const c16 *>(a.in) +
+                                                                  size_t(site) * 4 * N * a.k + size_t(slot) * R);
+  const size_t stride = size_t(a.k) / R;
+#pragma unroll
+  for (int sc = 0; sc < 4 * N; ++sc) {
+    CVec<c16, R> v = p[sc * stride];
+#pragma unroll
+    for (int r = 0; r < R; ++r) psi[r][sc / N][sc % N] = as_h2(v.v[r]);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void load_link_h2(h2 (&V)[N][N], const c16 *blk8, int slot)
+{
+  const c16 *p = blk8 + slot * N * N;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = as_h2(p[i * N + j]);
+}
+
+template <int D, int N, int R>
+__device__ __forceinline__ void sweep_h2(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
+                                         const c16 *blk8, h2 ss, float s, h2 (&psi_cur)[R][4][N],
+                                         h2 (&V_cur)[N][N], h2 (&psi_nxt)[R][4][N], h2 (&V_nxt)[N][N])
+{
+  if constexpr (D + 1 < 8) {
+    load_spinor_h2<N, R, float>(psi_nxt, a, g.nbr[D + 1], slot);
+    load_link_h2<N>(V_nxt, blk8, native_link_slot(D + 1));
+  }
+  if (g.live[D]) {
+    h2 Vc[N][N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) Vc[i][j] = link_c_h2(V_cur[i][j]);
+    float z;
+    asm("s_mov_b32 %0, 0" : "=s"(z));
+#pragma unroll
+    for (int r = 0; r < R; ++r) hop_one_h2<N, D>(acc[r], V_cur, Vc, psi_cur[r], ss, s, z);
+  }
+  if constexpr (D + 1 < 8) sweep_h2<D + 1, N, R>(acc, a, g, slot, blk8, ss, s, psi_nxt, V_nxt, psi_cur, V_cur);
+}
+
 // Checkerboard index of a logical site number.
 __device__ __forceinline__ uint32_t logical_to_phys(uint32_t sl, const DslashArgs &a)
 {
@@ -398,9 +527,16 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
       for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
-        for (int c = 0; c < N; ++c) acc[r][sp][c] = cz<A>();
+        for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};  // x + (-0) == x: folds
 
-    {
+    if constexpr (SL == SL_PACKED && GL == GL_NATIVE && std::is_same<CS, c16>::value &&
+                  std::is_same<CG, c16>::value) {
+      h2 psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
+      const h2 ss = {_Float16(a.sign), _Float16(a.sign)};
+      load_spinor_h2<N, R, float>(psi0, a, g.nbr[0], slot);
+      load_link_h2<N>(V0, blk8, native_link_slot(0));
+      sweep_h2<0, N, R>(acc, a, g, slot, blk8, ss, float(s), psi0, V0, psi1, V1);
+    } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
       fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);

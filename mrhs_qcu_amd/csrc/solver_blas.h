@@ -10,7 +10,7 @@ constexpr int kBlasMaxRhs = 256;
 
 // Per-RHS scalars, all device arrays of length k.
 struct CgState {
-  double *rr, *rr_new, *pap, *alpha, *beta, *pnorm2, *pnorm, *b2;
+  double *rr, *rr_new, *pap, *alpha, *beta, *pnorm2, *pnorm, *b2, *rdotp, *gamma;
   int *done, *sloppy_conv;
   float *sc_p, *sc_t, *sc_q, *sc_t2;  // sloppy storage scales (true = stored * sc)
 };
@@ -21,8 +21,11 @@ long long blas_num_blocks(long long n, int k);
 
 hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const void *p, const double *alpha,
                           double *partial, long long n, int k, hipStream_t st);
+// p = gamma r + beta p (gamma = NULL: 1) ; ps = p / sc_p ; partial |p|^2
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const float *sc_p, double *partial, long long n, int k, hipStream_t st);
+                         const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st);
+// partial Re <a, b>
+hipError_t blas_dot_re(int prec, const void *a, const void *b, double *partial, long long n, int k, hipStream_t st);
 hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
 hipError_t blas_residual(void *r, const void *b, const void *Ax, double *partial, long long n, int k, hipStream_t st);
 hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st);
@@ -30,5 +33,6 @@ hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
 hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st);
 hipError_t cg_alpha(const CgState &s, int k, hipStream_t st);
 hipError_t cg_beta(const CgState &s, int k, double stop2, hipStream_t st);
+hipError_t cg_orth(const CgState &s, int k, hipStream_t st);
 
 }  // namespace qmrhs

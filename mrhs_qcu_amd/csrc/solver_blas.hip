@@ -56,11 +56,13 @@ __global__ void __launch_bounds__(kBlasBlock) update_rx_kernel(CK *__restrict__ 
   block_partial(acc, rhs, k, partial);
 }
 
-// p = r + beta p ; p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2
+// p = gamma r + beta p (gamma null: 1) ; p_s = p / sc_p (sloppy copy, may be
+// null) ; partial |p|^2
 template <class CK, class CS>
 __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p, CS *__restrict__ ps,
                                                               const CK *__restrict__ r,
                                                               const double *__restrict__ beta,
+                                                              const double *__restrict__ gamma,
                                                               const float *__restrict__ sc_p, double *partial,
                                                               long long n, int k)
 {
@@ -69,15 +71,33 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = int(g % k);
   const A b = A(beta[rhs]);
+  const A c = gamma ? A(gamma[rhs]) : A(1);
   const float inv = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
   double acc = 0.0;
   for (long long e = g; e < n; e += T) {
     cpx<A> pp = ld_cvt<A>(p[e]), rr = ld_cvt<A>(r[e]);
-    pp.re = rr.re + b * pp.re;
-    pp.im = rr.im + b * pp.im;
+    pp.re = c * rr.re + b * pp.re;
+    pp.im = c * rr.im + b * pp.im;
     p[e] = StCvt<CK>::go(pp);
     if (ps) ps[e] = StCvt<CS>::go(cpx<float>{float(pp.re) * inv, float(pp.im) * inv});
     acc += double(pp.re) * pp.re + double(pp.im) * pp.im;
+  }
+  block_partial(acc, rhs, k, partial);
+}
+
+// partial Re <a, b> = sum Re(conj(a) b)
+template <class C>
+__global__ void __launch_bounds__(kBlasBlock) dot_re_kernel(const C *__restrict__ a, const C *__restrict__ b,
+                                                            double *partial, long long n, int k)
+{
+  using A = typename Acc<C>::T;
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs = int(g % k);
+  double acc = 0.0;
+  for (long long e = g; e < n; e += T) {
+    const cpx<A> x = ld_cvt<A>(a[e]), y = ld_cvt<A>(b[e]);
+    acc += double(x.re) * y.re + double(x.im) * y.im;
   }
   block_partial(acc, rhs, k, partial);
 }
@@ -212,15 +232,36 @@ __global__ void init_state_kernel(CgState s, int k)
   s.sc_t2[r] = 64.0f * sp;
 }
 
-// After a reliable update: restart the recurrence from the true residual and
-// mark the right-hand sides that meet the tolerance.
+// After a reliable update: the true residual norm takes the place of the
+// iterated one (the following beta uses it), and the right-hand sides that
+// meet the tolerance are marked done.
 __global__ void check_true_kernel(CgState s, const double *rr_true, int k, double stop2)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= k) return;
   const double rt = rr_true[r];
-  if (!s.done[r]) s.rr[r] = rt;
+  if (!s.done[r]) s.rr_new[r] = rt;
   if (rt <= stop2 * s.b2[r]) s.done[r] = 1;
+}
+
+// After a reliable update and cg_beta: the old search direction is made
+// orthogonal to the new (true) residual before it enters p = r + beta p, so
+// p = gamma r + beta p with gamma = 1 - beta <r, p> / |r|^2; the sloppy
+// storage scales follow the new bound |gamma| |r| + |beta| |p|.
+__global__ void orth_kernel(CgState s, int k)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double rr = s.rr[r];
+  const double gamma = (!s.done[r] && rr > 0.0) ? 1.0 - s.beta[r] * s.rdotp[r] / rr : 1.0;
+  s.gamma[r] = gamma;
+  const double bound = fabs(gamma) * sqrt(rr) + fabs(s.beta[r]) * s.pnorm[r];
+  float sp = float(bound / 1024.0);
+  if (!(sp > 1e-30f)) sp = 1e-30f;
+  s.sc_p[r] = sp;
+  s.sc_t[r] = 16.0f * sp;
+  s.sc_q[r] = 4.0f * sp;
+  s.sc_t2[r] = 64.0f * sp;
 }
 
 // Workgroup count: a multiple of q (the least q with q * kBlasBlock % k == 0,
@@ -255,24 +296,24 @@ hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const vo
 }
 
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const float *sc_p, double *partial, long long n, int k, hipStream_t st)
+                         const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st)
 {
   if (prec_k == PREC_DOUBLE) {
     hipLaunchKernelGGL((update_p_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(p), static_cast<c64 *>(ps),
-                       static_cast<const c64 *>(r), beta, sc_p, partial, n, k);
+                       static_cast<const c64 *>(r), beta, gamma, sc_p, partial, n, k);
   } else {
     switch (prec_s) {
       case PREC_HALF:
         hipLaunchKernelGGL((update_p_kernel<c32, c16>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c16 *>(ps),
-                           static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+                           static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
         break;
       case PREC_BF16:
         hipLaunchKernelGGL((update_p_kernel<c32, cb16>), QM_GRID(n, k), static_cast<c32 *>(p),
-                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
         break;
       default:
         hipLaunchKernelGGL((update_p_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c32 *>(ps),
-                           static_cast<const c32 *>(r), beta, sc_p, partial, n, k);
+                           static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
     }
   }
   return hipGetLastError();
@@ -293,6 +334,19 @@ hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int
     default:
       hipLaunchKernelGGL(norm2_kernel<cb16>, QM_GRID(n, k), static_cast<const cb16 *>(v), partial, n, k);
   }
+  return hipGetLastError();
+}
+
+hipError_t blas_dot_re(int prec, const void *a, const void *b, double *partial, long long n, int k, hipStream_t st)
+{
+  if (prec == PREC_DOUBLE)
+    hipLaunchKernelGGL(dot_re_kernel<c64>, QM_GRID(n, k), static_cast<const c64 *>(a), static_cast<const c64 *>(b),
+                       partial, n, k);
+  else if (prec == PREC_SINGLE)
+    hipLaunchKernelGGL(dot_re_kernel<c32>, QM_GRID(n, k), static_cast<const c32 *>(a), static_cast<const c32 *>(b),
+                       partial, n, k);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -350,6 +404,12 @@ hipError_t cg_alpha(const CgState &s, int k, hipStream_t st)
 hipError_t cg_beta(const CgState &s, int k, double stop2, hipStream_t st)
 {
   hipLaunchKernelGGL(beta_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, k, stop2);
+  return hipGetLastError();
+}
+
+hipError_t cg_orth(const CgState &s, int k, hipStream_t st)
+{
+  hipLaunchKernelGGL(orth_kernel, dim3((k + 63) / 64), dim3(64), 0, st, s, k);
   return hipGetLastError();
 }
 

@@ -259,6 +259,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   cs.pnorm2 = mem.alloc<double>(k, err, &ok);
   cs.pnorm = mem.alloc<double>(k, err, &ok);
   cs.b2 = mem.alloc<double>(k, err, &ok);
+  cs.rdotp = mem.alloc<double>(k, err, &ok);
+  cs.gamma = mem.alloc<double>(k, err, &ok);
   cs.done = mem.alloc<int>(k, err, &ok);
   cs.sloppy_conv = mem.alloc<int>(k, err, &ok);
   cs.sc_p = mem.alloc<float>(k, err, &ok);
@@ -300,7 +302,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     QM_HIP(hipMemcpyAsync(r, bhat, b64, hipMemcpyDeviceToDevice, st));
   }
   QM_HIP(blas_zero(pk, p, n, k, st));
-  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, sc_p, partial_p, n, k, st));
+  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, sc_p, partial_p, n, k, st));
 
   std::vector<int> done(k, 0);
   std::vector<double> rrt(k, 0.0), b2h(k, 0.0), rr_ru(k, 0.0);
@@ -312,14 +314,15 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // residual then cannot drift apart), or every live RHS reached the
   // tolerance, or reliable_interval iterations passed.
   const double delta = prm->reliable_delta > 0.0 ? prm->reliable_delta
-                       : (ps == PREC_SINGLE ? 1e-3 : (ps == PREC_HALF ? 1e-2 : 1e-1));
+                       : (ps == PREC_SINGLE ? 1e-3 : (ps == PREC_HALF ? 3e-3 : 1e-1));
   const double delta2 = mixed ? delta * delta : 0.0;
   // QCU_MRHS_CG_VERBOSE=1 prints the iterated and true residuals.
   const char *verb_env = getenv("QCU_MRHS_CG_VERBOSE");
   const bool verbose = verb_env && verb_env[0] == '1';
 
-  // Per-iteration flags come back through pinned host memory, read one
-  // iteration late so the device queue never drains while the host decides.
+  // The iterated residual norms come back through pinned host memory once per
+  // iteration; the host decides on a reliable update before the search
+  // direction is formed, so an update can re-orthogonalise it.
   struct Pinned {
     void *p = nullptr;
     ~Pinned()
@@ -327,22 +330,11 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       if (p) (void)hipHostFree(p);
     }
   } pin;
-  QM_HIP(hipHostMalloc(&pin.p, 2 * size_t(k) * (sizeof(double) + sizeof(int)), hipHostMallocDefault));
-  double *h_rr[2] = {static_cast<double *>(pin.p), static_cast<double *>(pin.p) + k};
-  int *h_conv[2] = {reinterpret_cast<int *>(h_rr[1] + k), reinterpret_cast<int *>(h_rr[1] + k) + k};
-  struct Events {
-    hipEvent_t e[2] = {nullptr, nullptr};
-    ~Events()
-    {
-      for (auto &x : e)
-        if (x) (void)hipEventDestroy(x);
-    }
-  } ev;
-  QM_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
-  QM_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+  QM_HIP(hipHostMalloc(&pin.p, size_t(k) * sizeof(double), hipHostMallocDefault));
+  double *h_rr = static_cast<double *>(pin.p);
 
-  // True residual: x_h += x_k (mixed), r_h = b^ - M^+M^ x_h, rr_true, done
-  // flags; the Krylov residual restarts from r_h.
+  // True residual: x_h += x_k (mixed), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces
+  // the iterated norm, done flags; the Krylov residual restarts from r_h.
   auto reliable_update = [&]() -> bool {
     if (mixed && !hip_ok(blas_accumulate_zero(pk, x_out, xk, n, k, st), err, "accumulate")) return false;
     if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
@@ -381,9 +373,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   converged = true;
   for (int i = 0; i < k; ++i) converged = converged && done[i];
 
-  bool pending = false;
   while (!converged && iter < max_iter) {
-    const int cur = iter & 1;
     // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
     QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
     QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
@@ -393,35 +383,36 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
     QM_TRY(s.dslash(ap, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, false));
-    // r -= alpha Ap ; x += alpha p ; beta ; p = r + beta p
+    // r -= alpha Ap ; x += alpha p
     QM_HIP(blas_update_rx(pk, r, xk, ap, p, cs.alpha, s.partial, n, k, st));
     QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.rr_new, st));
-    QM_HIP(cg_beta(cs, k, stop2, st));
-    QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, sc_p, partial_p, n, k, st));
-    QM_HIP(hipMemcpyAsync(h_rr[cur], cs.rr, k * sizeof(double), hipMemcpyDeviceToHost, st));
-    QM_HIP(hipMemcpyAsync(h_conv[cur], cs.sloppy_conv, k * sizeof(int), hipMemcpyDeviceToHost, st));
-    QM_HIP(hipEventRecord(ev.e[cur], st));
+    QM_HIP(hipMemcpyAsync(h_rr, cs.rr_new, k * sizeof(double), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
     ++iter;
     ++since_update;
-    if (!pending) {
-      pending = true;
-      continue;
-    }
-    // Decide on the previous iteration's flags while this one runs.
-    const int prv = cur ^ 1;
-    QM_HIP(hipEventSynchronize(ev.e[prv]));
     bool all_conv = true, drop = false;
     double mx = 0;
     for (int i = 0; i < k; ++i) {
       if (done[i]) continue;
-      all_conv = all_conv && h_conv[prv][i];
-      drop = drop || (h_rr[prv][i] < delta2 * rr_ru[i]);
-      mx = std::max(mx, b2h[i] > 0 ? std::sqrt(h_rr[prv][i] / b2h[i]) : 0.0);
+      all_conv = all_conv && h_rr[i] <= stop2 * b2h[i];
+      drop = drop || (h_rr[i] < delta2 * rr_ru[i]);
+      mx = std::max(mx, b2h[i] > 0 ? std::sqrt(h_rr[i] / b2h[i]) : 0.0);
     }
-    if (verbose && iter % 10 == 0) fprintf(stderr, "cg iter %d: iterated relative residual max %.3e\n", iter - 1, mx);
+    if (verbose && iter % 10 == 0) fprintf(stderr, "cg iter %d: iterated relative residual max %.3e\n", iter, mx);
     if (all_conv || drop || (mixed && since_update >= interval) || iter == max_iter) {
       QM_TRY(reliable_update());
-      pending = false;  // flags queued before the update are stale
+      if (converged || iter == max_iter) break;
+      // beta = |r_true|^2 / |r_old|^2 ; p = gamma r + beta p with the old p
+      // made orthogonal to r_true.
+      QM_HIP(blas_dot_re(pk, r, p, s.partial, n, k, st));
+      QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.rdotp, st));
+      QM_HIP(cg_beta(cs, k, stop2, st));
+      QM_HIP(cg_orth(cs, k, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, sc_p, partial_p, n, k, st));
+    } else {
+      // beta = |r|^2 / |r_old|^2 ; p = r + beta p
+      QM_HIP(cg_beta(cs, k, stop2, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, sc_p, partial_p, n, k, st));
     }
   }
   if (!converged && since_update > 0) QM_TRY(reliable_update());
