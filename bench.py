@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--cg-L", type=int, nargs=4, default=[32, 32, 32, 64])
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
     ap.add_argument("--tile", type=str, default=None,
-                    help="fixed tile R,block,lds,tblock[,zblock,hints] for the headline kernel (skips the tuner)")
+                    help="fixed tile R,block,lds,tblock[,zblock,hints,occupancy,traversal] for the headline kernel "
+                         "(skips the tuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
     ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
@@ -161,6 +162,64 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
     return out
 
 
+BATCH_WORKLOAD = [((16, 16, 16, 16), 2, 8), ((24, 24, 24, 48), 3, 12), ((16, 16, 16, 32), 4, 24), ((12, 12, 12, 24), 5, 6)]
+
+
+def batch_benchmark(lib, _lib, torch, steps=50, warmup=5):
+    """Uneven batched workload in one HIP graph (SU(2) 16^4 nrhs=8, SU(3)
+    24^3x48 nrhs=12, SU(4) 16^3x32 nrhs=24, SU(5) 12^3x24 nrhs=6, FP32, tuned
+    tile per problem): time per graph replay against the same launches issued
+    one after another on one stream."""
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    probs = (_lib.Problem * len(BATCH_WORKLOAD))()
+    keep, tot_sites_rhs, tot_bytes = [], 0, 0
+    for i, (L, N, k) in enumerate(BATCH_WORKLOAD):
+        geom = _lib.geom(L, N)
+        S = fields.half_volume(L)
+        gu = torch.view_as_real(fields.gauge_field_device(L, N, seed=100 + i, device=dev)).reshape(-1)
+        gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, _lib.SINGLE) // 4, dtype=torch.float32, device=dev)
+        _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), _lib.SINGLE, gu.data_ptr(), _lib.DOUBLE, geom, sp))
+        del gu
+        a = torch.randn(S * 4 * N * k * 2, device=dev, dtype=torch.float32)
+        o = torch.empty_like(a)
+        probs[i].geom = geom
+        probs[i].nrhs, probs[i].parity, probs[i].dagger = k, i & 1, 0
+        probs[i].out, probs[i].in_, probs[i].gauge = o.data_ptr(), a.data_ptr(), gn.data_ptr()
+        keep += [gn, a, o]
+        tot_sites_rhs += S * k
+        tot_bytes += bytes_alg(S, k, N, 4, 4, 4)
+    h = ctypes.c_void_p()
+    _lib.check(lib.qcuMrhsBatchCreate(probs, len(BATCH_WORKLOAD), ctypes.byref(h), sp))
+    try:
+        for _ in range(warmup):
+            _lib.check(lib.qcuMrhsBatchRun(h, sp))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            lib.qcuMrhsBatchRun(h, sp)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms_graph = e0.elapsed_time(e1) / steps
+        e0.record(st)
+        for _ in range(steps):
+            for p in probs:
+                lib.qcuMrhsDslash(p.out, p.in_, p.gauge, _lib.SINGLE, p.nrhs, p.parity, p.dagger, p.geom, None, sp)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms_seq = e0.elapsed_time(e1) / steps
+    finally:
+        lib.qcuMrhsBatchDestroy(h)
+    return {"workload": "batch su2_16^4_k8 + su3_24^3x48_k12 + su4_16^3x32_k24 + su5_12^3x24_k6 fp32",
+            "ms_per_graph_replay": ms_graph, "ms_sequential_launches": ms_seq,
+            "sites_rhs_per_s": tot_sites_rhs / (ms_graph * 1e-3),
+            "hbm_frac": tot_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_alg_per_replay": tot_bytes}
+
+
 def cpu_baseline(L, N, k, budget_s=12.0):
     """FP32 C oracle (OpenMP) on the host: same workload, bounded repetitions."""
     import numpy as np
@@ -254,7 +313,8 @@ def main():
                   "fp64": "f64"}[args.precision],
         "data": "synthetic (seeded SU(3) links, Gaussian spinors)",
         "config": {"workload": wl, "lattice": list(L), "nColors": N, "nrhs": k, "storage": args.precision,
-                   "tile": dict(zip(("rhs_per_lane", "block", "link_lds", "t_block", "z_block", "cache_hints"), tile)),
+                   "tile": dict(zip(("rhs_per_lane", "block", "link_lds", "t_block", "z_block", "cache_hints",
+                                     "occupancy", "traversal"), tile)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(f"{wl}_traffic.json"),
@@ -275,6 +335,7 @@ def main():
             "ms_per_step": ms3, "sites_rhs_per_s": S2 * kc2 / (ms3 * 1e-3),
             "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tile": t3}
+        extra["batch"] = batch_benchmark(lib, _lib, torch)
         line["extra"] = extra
         if not args.no_cg:
             line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))

@@ -50,6 +50,8 @@ typedef struct qcuMrhsTile_s {
   int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads;
                        bit 2: keep the hardware workgroup order (no XCD-contiguous remap) */
   int occupancy;    /* cap on resident workgroups per CU (3..8), 0 = no cap */
+  int traversal;    /* 0: each (t_block, z_block) block in (t, z, y, x) order;
+                       1: the block streamed along y, (y, t, z, x) order inside it */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);

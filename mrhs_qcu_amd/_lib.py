@@ -28,11 +28,11 @@ class Geom(ctypes.Structure):
 class Tile(ctypes.Structure):
     _fields_ = [("rhs_per_lane", ctypes.c_int), ("block", ctypes.c_int), ("link_lds", ctypes.c_int),
                 ("t_block", ctypes.c_int), ("z_block", ctypes.c_int), ("cache_hints", ctypes.c_int),
-                ("occupancy", ctypes.c_int)]
+                ("occupancy", ctypes.c_int), ("traversal", ctypes.c_int)]
 
     def as_tuple(self):
         return (self.rhs_per_lane, self.block, self.link_lds, self.t_block, self.z_block, self.cache_hints,
-                self.occupancy)
+                self.occupancy, self.traversal)
 
 
 class CgParam(ctypes.Structure):
@@ -142,8 +142,8 @@ def ptr_array(ptrs):
     return ctypes.cast(arr, PP), arr
 
 
-def tile(R=1, block=256, lds=0, tblock=1, zblock=0, hints=0, occ=0):
+def tile(R=1, block=256, lds=0, tblock=1, zblock=0, hints=0, occ=0, traversal=0):
     t = Tile()
     t.rhs_per_lane, t.block, t.link_lds, t.t_block, t.z_block, t.cache_hints = R, block, lds, tblock, zblock, hints
-    t.occupancy = occ
+    t.occupancy, t.traversal = occ, traversal
     return t

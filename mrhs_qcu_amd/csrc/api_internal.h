@@ -20,6 +20,7 @@ Tile tile_for(const Geom &g, int prec, int k);
 Tile tile_from_c(const qcuMrhsTile &c);
 qcuMrhsTile tile_to_c(const Tile &t);
 void set_tile(const Geom &g, int prec, int k, const Tile &t);
+bool tuned(const Geom &g, int prec, int k);
 std::vector<Tile> tile_candidates(const Geom &g, int prec, int k);
 
 int run_packed_dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec,

@@ -1,0 +1,177 @@
+// Batched hopping term: several independent problems (own lattice, colour
+// count N, nrhs, parity, fields) applied by one replayable HIP graph.
+//
+// Create tunes every problem shape that has no tile yet (timing it on the
+// problem's own fields, so `out` is scratch until the first Run), freezes the
+// tile of each problem and captures one kernel node per problem. The nodes
+// hang off a common fork, so the runtime is free to run small problems side
+// by side; Run is a single hipGraphLaunch. The graph holds the field pointers
+// given at Create: the caller keeps those buffers alive and unmoved until
+// Destroy. Nothing in the graph clears memory (no memset nodes).
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+#include "qcu_mrhs.h"
+
+namespace qmrhs {
+namespace {
+
+struct Batch {
+  std::vector<qcuMrhsProblem> probs;
+  std::vector<Tile> tiles;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~Batch()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+// Streams and events that exist only while the graph is captured.
+struct CaptureScratch {
+  hipStream_t cap = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
+  bool capturing = false;
+  ~CaptureScratch()
+  {
+    if (capturing) {
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(cap, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    for (auto s : side) (void)hipStreamDestroy(s);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+DslashCall problem_call(const qcuMrhsProblem &p, const Geom &g, const Tile &t, hipStream_t st)
+{
+  DslashCall c;
+  c.sl = SL_PACKED;
+  c.gl = GL_NATIVE;
+  c.prec_s = c.prec_g = c.prec_o = PREC_SINGLE;
+  c.geom = g;
+  c.tile = t;
+  c.k = p.nrhs;
+  c.parity = p.parity;
+  c.dagger = p.dagger;
+  c.out = p.out;
+  c.in = p.in;
+  c.gauge = p.gauge;
+  c.stream = st;
+  return c;
+}
+
+}  // namespace
+}  // namespace qmrhs
+
+using namespace qmrhs;
+
+extern "C" {
+
+int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, void *stream)
+{
+  if (!handle) return api_fail("null handle pointer");
+  *handle = nullptr;
+  if (!probs || nprob < 1) return api_fail("a batch needs at least one problem");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto b = new Batch();
+  std::string err;
+  std::vector<Geom> geoms(nprob);
+  for (int i = 0; i < nprob; ++i) {
+    const qcuMrhsProblem &p = probs[i];
+    if (!geom_from_c(&p.geom, &geoms[i], &err)) {
+      delete b;
+      return api_fail("problem " + std::to_string(i) + ": " + err);
+    }
+    if (p.nrhs < 1 || !p.out || !p.in || !p.gauge) {
+      delete b;
+      return api_fail("problem " + std::to_string(i) + ": null field or nrhs < 1");
+    }
+    // Tune shapes seen for the first time (on this problem's own fields).
+    if (!tuned(geoms[i], PREC_SINGLE, p.nrhs)) {
+      qcuMrhsTile best;
+      double us = 0;
+      if (qcuMrhsTune(p.out, p.in, p.gauge, PREC_SINGLE, p.nrhs, &p.geom, &best, &us, stream)) {
+        const std::string m = qcuMrhsLastError();
+        delete b;
+        return api_fail("problem " + std::to_string(i) + ": tuning failed: " + m);
+      }
+    }
+    b->probs.push_back(p);
+    b->tiles.push_back(tile_for(geoms[i], PREC_SINGLE, p.nrhs));
+  }
+  // Every launch is validated eagerly first, so a bad problem fails here and
+  // not inside the capture.
+  for (int i = 0; i < nprob; ++i) {
+    if (launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], st), &err)) {
+      delete b;
+      return api_fail("problem " + std::to_string(i) + ": " + err);
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    delete b;
+    return api_fail("batch validation launches failed");
+  }
+  {
+    CaptureScratch cs;
+    bool ok = hipStreamCreateWithFlags(&cs.cap, hipStreamNonBlocking) == hipSuccess;
+    cs.side.assign(nprob, nullptr);
+    cs.ev.assign(nprob + 1, nullptr);
+    for (int i = 0; ok && i < nprob; ++i) ok = hipStreamCreateWithFlags(&cs.side[i], hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; ok && i <= nprob; ++i) ok = hipEventCreateWithFlags(&cs.ev[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      delete b;
+      return api_fail("batch: stream or event creation failed");
+    }
+    ok = hipStreamBeginCapture(cs.cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    cs.capturing = ok;
+    ok = ok && hipEventRecord(cs.ev[nprob], cs.cap) == hipSuccess;
+    for (int i = 0; ok && i < nprob; ++i) {
+      ok = hipStreamWaitEvent(cs.side[i], cs.ev[nprob], 0) == hipSuccess &&
+           !launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.side[i]), &err) &&
+           hipEventRecord(cs.ev[i], cs.side[i]) == hipSuccess && hipStreamWaitEvent(cs.cap, cs.ev[i], 0) == hipSuccess;
+    }
+    hipGraph_t g = nullptr;
+    if (cs.capturing) {
+      const bool ended = hipStreamEndCapture(cs.cap, &g) == hipSuccess;
+      cs.capturing = false;
+      ok = ok && ended && g;
+      if (!ok && g) (void)hipGraphDestroy(g);
+    }
+    if (!ok) {
+      delete b;
+      return api_fail("batch: graph capture failed" + (err.empty() ? std::string() : ": " + err));
+    }
+    b->graph = g;
+  }
+  if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+    delete b;
+    return api_fail("batch: graph instantiation failed");
+  }
+  *handle = b;
+  return 0;
+}
+
+int qcuMrhsBatchRun(void *handle, void *stream)
+{
+  if (!handle) return api_fail("null batch handle");
+  Batch *b = static_cast<Batch *>(handle);
+  const hipError_t e = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return api_fail(std::string("batch launch failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int qcuMrhsBatchDestroy(void *handle)
+{
+  if (!handle) return api_fail("null batch handle");
+  delete static_cast<Batch *>(handle);
+  return 0;
+}
+
+}  // extern "C"

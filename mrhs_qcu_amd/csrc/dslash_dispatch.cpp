@@ -124,6 +124,8 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.div_tb = make_fastdiv(uint32_t(a.tblock));
   a.div_zb = make_fastdiv(uint32_t(a.zblock));
   a.div_nzb = make_fastdiv(uint32_t(a.Lz / a.zblock));
+  a.div_Ly = make_fastdiv(uint32_t(a.Ly));
+  a.ystream = c.tile.ystream ? 1 : 0;
   a.div_S = make_fastdiv(uint32_t(a.S));
 }
 
@@ -216,6 +218,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     b.div_tb = make_fastdiv(1);
     b.div_zb = make_fastdiv(uint32_t(b.Lz));
     b.div_nzb = make_fastdiv(1);
+    b.ystream = 0;
     const unsigned long long lanes = 1ULL * b.S * kk;
     if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
     b.nlanes = uint32_t(lanes);

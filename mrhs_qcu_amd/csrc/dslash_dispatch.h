@@ -73,6 +73,7 @@ struct Tile {
   int hints = 0;   // bit 0: non-temporal output stores, bit 1: non-temporal link loads,
                    // bit 2: keep the hardware workgroup order (no XCD remap)
   int occ = 0;     // cap on resident workgroups per CU (LDS padding); 0 = no cap
+  int ystream = 0; // 1: stream each (tblock, zblock) block along y
 };
 
 struct DslashCall {

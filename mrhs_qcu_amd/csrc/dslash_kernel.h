@@ -65,7 +65,8 @@ struct DslashArgs {
   int xcd_map;            // remap workgroups so each XCD owns a contiguous range
   uint32_t nlanes;        // S * P
   uint32_t nblocks;
-  FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S;
+  FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S, div_Ly;
+  int ystream;  // traversal (t_out, z_out, y, t_in, z_in, x/2) instead of (t_out, z_out, t_in, z_in, y, x/2)
 };
 
 // Hop table: direction D = 2*mu + (backward); projection partner spins a0/a1
@@ -168,11 +169,29 @@ struct SiteGeo {
   int live[8];   // 0 if the term is skipped at a local boundary
 };
 
-// Logical site number -> coordinates for the traversal order
-// (t_out, z_out, t_in, z_in, y, x/2).
+// Logical site number -> coordinates for the traversal orders
+//   (t_out, z_out, t_in, z_in, y, x/2)   blocks of t_block x z_block slices, or
+//   (t_out, z_out, y, t_in, z_in, x/2)   the same blocks streamed along y, so
+// that an XCD's L2 holds the three y planes of a block (with their t and z
+// halo) that the neighbour loads of one plane touch.
 __device__ __forceinline__ void decode_site(uint32_t sl, const DslashArgs &a, uint32_t &xh, uint32_t &y, uint32_t &z,
                                             uint32_t &t)
 {
+  if (a.ystream) {
+    const uint32_t q = fdiv(sl, a.div_subLx);
+    xh = sl - q * a.subLx;
+    const uint32_t q2 = fdiv(q, a.div_zb);
+    const uint32_t zin = q - q2 * a.zblock;
+    const uint32_t q3 = fdiv(q2, a.div_tb);
+    const uint32_t tin = q2 - q3 * a.tblock;
+    const uint32_t q4 = fdiv(q3, a.div_Ly);
+    y = q3 - q4 * a.Ly;
+    const uint32_t tout = fdiv(q4, a.div_nzb);
+    const uint32_t zout = q4 - tout * (a.Lz / a.zblock);
+    z = zout * a.zblock + zin;
+    t = tout * a.tblock + tin;
+    return;
+  }
   const uint32_t row = uint32_t(a.subLx) * a.Ly;
   const uint32_t q = fdiv(sl, a.div_row);
   const uint32_t r = sl - q * row;

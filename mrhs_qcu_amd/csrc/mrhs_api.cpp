@@ -153,6 +153,21 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
             }
           }
         }
+        // Blocks streamed along y (t_block x z_block cross-sections).
+        for (int tb : {2, 3, 4, 6}) {
+          if (g.L[3] % tb || tb >= g.L[3]) continue;
+          for (int zb : {2, 3, 4, 6, 8}) {
+            if (g.L[2] % zb || zb >= g.L[2]) continue;
+            Tile t;
+            t.R = R;
+            t.lds = lds;
+            t.block = block;
+            t.tblock = tb;
+            t.zblock = zb;
+            t.ystream = 1;
+            out.push_back(t);
+          }
+        }
       }
     }
   }
@@ -169,6 +184,7 @@ Tile tile_from_c(const qcuMrhsTile &c)
   t.zblock = c.z_block;
   t.hints = c.cache_hints;
   t.occ = c.occupancy;
+  t.ystream = c.traversal;
   return t;
 }
 
@@ -182,6 +198,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
   c.occupancy = t.occ;
+  c.traversal = t.ystream;
   return c;
 }
 

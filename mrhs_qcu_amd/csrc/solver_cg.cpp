@@ -178,7 +178,6 @@ bool hip_ok(hipError_t e, std::string *err, const char *what)
 
 }  // namespace
 
-bool tuned(const Geom &g, int prec, int k);
 
 static bool tune_now(void *out, void *in, const void *gauge, int prec, int k, const Geom &g, hipStream_t st,
                      std::string *err)

@@ -1,6 +1,7 @@
 """CPU checks of the C-ABI library: it loads, exports every symbol the public
 headers declare, validates arguments without touching the GPU, and keeps the
 reference's fatal-error convention (message on stderr, exit status 1)."""
+import ctypes
 import os
 import re
 import subprocess
@@ -66,6 +67,11 @@ def test_argument_validation_without_gpu(lib):
     assert "aligned" in lib.qcuMrhsLastError().decode()
     assert lib.qcuMrhsDslash(16, 16, 16, 3, 2, 0, 0, g, _lib.tile(), None) != 0
     assert lib.qcuMrhsDslash(16, 16, 16, _lib.SINGLE, 2, 2, 0, g, _lib.tile(), None) != 0
+    h = ctypes.c_void_p()
+    assert lib.qcuMrhsBatchCreate(None, 0, ctypes.byref(h), None) != 0
+    assert "at least one problem" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsBatchRun(None, None) != 0 and lib.qcuMrhsBatchDestroy(None) != 0
+    assert "null batch handle" in lib.qcuMrhsLastError().decode()
 
 
 DEATH = [

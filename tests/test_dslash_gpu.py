@@ -5,6 +5,7 @@ FP64 1e-13, FP32 1e-6; FP16 / BF16 storage with FP32 arithmetic are stated
 (printed) and bounded loosely (3e-3 / 2e-2).
 """
 import ctypes
+import itertools
 
 import numpy as np
 import pytest
@@ -109,8 +110,8 @@ def test_colors(lib, dev, N, prec):
 
 
 def test_every_tile_agrees(lib, dev):
-    """Every instantiated tile (R, LDS staging, block, t-blocking) gives the
-    oracle's result (FP32, N = 3, k = 8 and k = 6)."""
+    """Every instantiated tile (R, LDS staging, block, t/z blocking, both
+    traversal orders) gives the oracle's result (FP32, N = 3, k = 8 and 6)."""
     L, N = (8, 4, 6, 8), 3
     g = fields.gauge_field(L, N, seed=3)
     for k in (8, 6):
@@ -121,21 +122,21 @@ def test_every_tile_agrees(lib, dev):
         for R in (1, 2, 4):
             for lds in (0, 1):
                 for block in (64, 128, 192, 256):
-                    for tb, zb in ((1, 0), (2, 0), (4, 0), (2, 1), (4, 2), (2, 3)):
+                    for (tb, zb), trav in itertools.product(((1, 0), (2, 0), (4, 0), (2, 1), (4, 2), (2, 3)), (0, 1)):
                         if k % R or L[3] % tb:
                             continue
                         hints = (block // 64) & 3
                         pout = torch.zeros_like(pin)
                         rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 1, 0,
-                                               geom, _lib.tile(R, block, lds, tb, zb, hints), stream())
+                                               geom, _lib.tile(R, block, lds, tb, zb, hints, 0, trav), stream())
                         if rc != 0:
                             msg = lib.qcuMrhsLastError().decode()
                             assert "no kernel instantiated" in msg or "LDS" in msg, msg
                             continue
                         got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
-                        assert rel_err(got, want) < TOL[1], (R, lds, block, tb, zb, hints)
+                        assert rel_err(got, want) < TOL[1], (R, lds, block, tb, zb, hints, trav)
                         n += 1
-        assert n >= 48
+        assert n >= 96
 
 
 @pytest.mark.parametrize("grid", [(2, 1, 1, 1), (1, 1, 1, 2), (2, 2, 2, 2)])
