@@ -106,6 +106,10 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.xa = c.xa;
   a.xb = c.xb;
   a.partial = c.partial;
+  a.rx_r = c.rx_r;
+  a.rx_x = c.rx_x;
+  a.rx_p = c.rx_p;
+  a.rx_alpha = c.rx_alpha;
   a.sign = c.dagger ? -1.0f : 1.0f;
   a.parity = c.parity;
   a.subLx = c.geom.L[0] / 2;
@@ -163,6 +167,13 @@ int launch_dslash(const DslashCall &c, std::string *err)
     if ((uintptr_t(c.out) | uintptr_t(c.in) | uintptr_t(c.gauge) | uintptr_t(c.x)) % al)
       return fail(err, "packed fields must be 16-byte aligned"), 1;
     if (c.partial && c.k > 256) return fail(err, "norm partials support at most 256 RHS"), 1;
+    if (c.rx_r) {
+      if (c.prec_o != PREC_SINGLE && c.prec_o != PREC_DOUBLE)
+        return fail(err, "the fused CG update needs FP32 or FP64 output"), 1;
+      if (!c.rx_x || !c.rx_p || !c.rx_alpha) return fail(err, "fused CG update: null r, x, p or alpha"), 1;
+      if ((uintptr_t(c.rx_r) | uintptr_t(c.rx_x) | uintptr_t(c.rx_p)) % al)
+        return fail(err, "packed fields must be 16-byte aligned"), 1;
+    }
     if (c.gl != GL_NATIVE) return fail(err, "packed spinors use the native gauge layout"), 1;
     const int lds = c.tile.lds ? 1 : 0;
     KernelKey key{SL_PACKED, GL_NATIVE, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
@@ -194,7 +205,8 @@ int launch_dslash(const DslashCall &c, std::string *err)
 
   // Pointer layouts: one RHS per lane, at most kMaxPtrs RHS per launch.
   if (!c.out_ptrs || !c.in_ptrs) return fail(err, "null pointer arrays"), 1;
-  if (c.x || c.sc_out || c.sc_x || c.partial) return fail(err, "xpay / scaling / norms need the packed layout"), 1;
+  if (c.x || c.sc_out || c.sc_x || c.partial || c.rx_r)
+    return fail(err, "xpay / scaling / norms need the packed layout"), 1;
   const bool sm = c.sl == SL_SOA_PTRS;
   if (c.sl == SL_AOS_PTRS && c.gl != GL_NATIVE) return fail(err, "AoS spinors use the native gauge layout"), 1;
   if (c.sl == SL_SOA_PTRS && c.gl != GL_REF_SOA) return fail(err, "SoA spinors use the reference SoA gauge"), 1;

@@ -89,6 +89,10 @@ struct DslashCall {
   const float *sc_in = nullptr, *sc_x = nullptr, *sc_out = nullptr;
   double xa = 0.0, xb = 1.0;
   double *partial = nullptr;  // [nblocks][k]; nblocks from dslash_num_blocks()
+  // Fused CG update r -= alpha out, x += alpha p (FP32 packed output only).
+  void *rx_r = nullptr, *rx_x = nullptr;
+  const void *rx_p = nullptr;
+  const double *rx_alpha = nullptr;
   hipStream_t stream = nullptr;
 };
 

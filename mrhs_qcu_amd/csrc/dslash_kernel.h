@@ -51,6 +51,13 @@ struct DslashArgs {
   const float *sc_out;    // per-RHS: stored out = true out / sc_out[r]         (null = 1)
   double xa, xb;          // true out = xa * x + xb * D(in)   (x null: xb * D(in))
   double *partial;        // per-workgroup per-RHS sum of |true out|^2, [nblocks][k] (nullable)
+  // Fused CG update (packed layout, FP32 or FP64 output; rx_r null: off):
+  // with v = the true out, r -= alpha v and x += alpha p in place (packed
+  // fields of the output type), out is not written and partial receives
+  // |r|^2 instead.
+  void *rx_r, *rx_x;
+  const void *rx_p;
+  const double *rx_alpha;
   float sign;             // +1 (daggerFlag 0) or -1 (daggerFlag 1)
   int k;                  // RHS count (RHS stride of the packed layout)
   int P;                  // lanes per site = k / R
@@ -571,7 +578,50 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
       fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
     }
-    if constexpr (SL == SL_PACKED) {
+    bool fused = false;
+    if constexpr (SL == SL_PACKED && ((std::is_same<CO, c32>::value && std::is_same<A, float>::value) ||
+                                      (std::is_same<CO, c64>::value && std::is_same<A, double>::value))) {
+      if (a.rx_r) {
+        fused = true;
+        const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
+        const size_t stride = size_t(a.k) / R;
+        CVec<CO, R> *pr = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_r) + base);
+        CVec<CO, R> *pxk = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_x) + base);
+        const CVec<CO, R> *pp = reinterpret_cast<const CVec<CO, R> *>(static_cast<const CO *>(a.rx_p) + base);
+        const CVec<CS, R> *px =
+            a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
+        A al[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) al[r] = A(a.rx_alpha[int(slot) * R + r]);
+#pragma unroll
+        for (int sc = 0; sc < 4 * N; ++sc) {
+          CVec<CS, R> xv;
+          if (px) xv = px[sc * stride];
+          CVec<CO, R> rv = pr[sc * stride], xk = pxk[sc * stride];
+          const CVec<CO, R> pv = pp[sc * stride];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            cpx<A> v = acc[r][sc / N][sc % N];
+            v.re *= fin[r];
+            v.im *= fin[r];
+            if (px) {
+              const cpx<A> xx = ld_cvt<A>(xv.v[r]);
+              v.re += fx[r] * xx.re;
+              v.im += fx[r] * xx.im;
+            }
+            rv.v[r].x -= al[r] * v.re;
+            rv.v[r].y -= al[r] * v.im;
+            xk.v[r].x += al[r] * pv.v[r].x;
+            xk.v[r].y += al[r] * pv.v[r].y;
+            nrm[r] += rv.v[r].x * rv.v[r].x + rv.v[r].y * rv.v[r].y;
+          }
+          pr[sc * stride] = rv;
+          pxk[sc * stride] = xk;
+        }
+      }
+    }
+    if (fused) {
+    } else if constexpr (SL == SL_PACKED) {
       const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
       const size_t stride = size_t(a.k) / R;
       CVec<CO, R> *po = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.out) + base);

@@ -19,8 +19,6 @@ struct CgState {
 // partial-sum buffers hold blas_num_blocks(n, k) * k doubles).
 long long blas_num_blocks(long long n, int k);
 
-hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const void *p, const double *alpha,
-                          double *partial, long long n, int k, hipStream_t st);
 // p = gamma r + beta p (gamma = NULL: 1) ; ps = p / sc_p ; partial |p|^2
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
                          const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st);

@@ -30,32 +30,6 @@ __device__ __forceinline__ void block_partial(double v, int rhs, int k, double *
   for (int i = threadIdx.x; i < k; i += blockDim.x) partial[size_t(blockIdx.x) * k + i] = red[i];
 }
 
-// r -= alpha Ap ; x += alpha p ; partial |r|^2
-template <class CK>
-__global__ void __launch_bounds__(kBlasBlock) update_rx_kernel(CK *__restrict__ r, CK *__restrict__ x,
-                                                               const CK *__restrict__ Ap, const CK *__restrict__ p,
-                                                               const double *__restrict__ alpha, double *partial,
-                                                               long long n, int k)
-{
-  using A = typename Acc<CK>::T;
-  const long long T = (long long)gridDim.x * blockDim.x;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = int(g % k);
-  const A a = A(alpha[rhs]);
-  double acc = 0.0;
-  for (long long e = g; e < n; e += T) {
-    cpx<A> rr = ld_cvt<A>(r[e]), ap = ld_cvt<A>(Ap[e]), xx = ld_cvt<A>(x[e]), pp = ld_cvt<A>(p[e]);
-    rr.re -= a * ap.re;
-    rr.im -= a * ap.im;
-    xx.re += a * pp.re;
-    xx.im += a * pp.im;
-    r[e] = StCvt<CK>::go(rr);
-    x[e] = StCvt<CK>::go(xx);
-    acc += double(rr.re) * rr.re + double(rr.im) * rr.im;
-  }
-  block_partial(acc, rhs, k, partial);
-}
-
 // p = gamma r + beta p (gamma null: 1) ; p_s = p / sc_p (sloppy copy, may be
 // null) ; partial |p|^2
 template <class CK, class CS>
@@ -282,18 +256,6 @@ unsigned grid_for(long long n, int k)
 long long blas_num_blocks(long long n, int k) { return grid_for(n, k); }
 
 #define QM_GRID(n, k) dim3(grid_for(n, k)), dim3(kBlasBlock), 0, st
-
-hipError_t blas_update_rx(int prec_k, void *r, void *x, const void *Ap, const void *p, const double *alpha,
-                          double *partial, long long n, int k, hipStream_t st)
-{
-  if (prec_k == PREC_DOUBLE)
-    hipLaunchKernelGGL(update_rx_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<c64 *>(x),
-                       static_cast<const c64 *>(Ap), static_cast<const c64 *>(p), alpha, partial, n, k);
-  else
-    hipLaunchKernelGGL(update_rx_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(r), static_cast<c32 *>(x),
-                       static_cast<const c32 *>(Ap), static_cast<const c32 *>(p), alpha, partial, n, k);
-  return hipGetLastError();
-}
 
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
                          const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st)

@@ -114,9 +114,15 @@ struct Solver {
 
   bool dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec_in,
               int prec_out, int parity, int dagger, const float *sc_in, const float *sc_x, const float *sc_out,
-              bool norm)
+              bool norm, const DslashCall *fuse = nullptr)
   {
     DslashCall c;
+    if (fuse) {
+      c.rx_r = fuse->rx_r;
+      c.rx_x = fuse->rx_x;
+      c.rx_p = fuse->rx_p;
+      c.rx_alpha = fuse->rx_alpha;
+    }
     c.sl = SL_PACKED;
     c.gl = GL_NATIVE;
     c.prec_s = prec_in;
@@ -235,7 +241,6 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   void *ax64 = mem.bytes(b64, err, &ok);
   void *r = mem.bytes(bk, err, &ok);
   void *p = mem.bytes(bk, err, &ok);
-  void *ap = mem.bytes(bk, err, &ok);
   void *xk = mixed ? mem.bytes(bk, err, &ok) : x_out;
   void *p_s = mixed ? mem.bytes(bs, err, &ok) : p;
   void *t_s = mixed ? mem.bytes(bs, err, &ok) : t64;
@@ -372,6 +377,12 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   converged = true;
   for (int i = 0; i < k; ++i) converged = converged && done[i];
 
+  // The r / x update is fused into the last operator launch of an iteration.
+  DslashCall fuse;
+  fuse.rx_r = r;
+  fuse.rx_x = xk;
+  fuse.rx_p = p;
+  fuse.rx_alpha = cs.alpha;
   while (!converged && iter < max_iter) {
     // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
     QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
@@ -381,10 +392,10 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     QM_HIP(cg_alpha(cs, k, st));
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
-    QM_TRY(s.dslash(ap, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, false));
-    // r -= alpha Ap ; x += alpha p
-    QM_HIP(blas_update_rx(pk, r, xk, ap, p, cs.alpha, s.partial, n, k, st));
-    QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.rr_new, st));
+    // Ap = q - kappa^2 D^+_eo t2 stays in registers: the same launch does
+    // r -= alpha Ap, x += alpha p and the per-RHS |r|^2 partials (out unused).
+    QM_TRY(s.dslash(r, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
+    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.rr_new, st));
     QM_HIP(hipMemcpyAsync(h_rr, cs.rr_new, k * sizeof(double), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));
     ++iter;
