@@ -3,13 +3,15 @@
 //
 // Create tunes every problem shape that has no tile yet (timing it on the
 // problem's own fields, so `out` is scratch until the first Run), freezes the
-// tile of each problem and captures one kernel node per problem. The nodes
-// hang off a common fork, so the runtime is free to run small problems side
-// by side; Run is a single hipGraphLaunch. The graph holds the field pointers
+// tile of each problem and captures one kernel node per problem, largest
+// first, as a chain: each launch already fills the GPU (>= 256 workgroups for
+// every shape of interest), and independent branches measured slower than the
+// chain. Run is a single hipGraphLaunch. The graph holds the field pointers
 // given at Create: the caller keeps those buffers alive and unmoved until
 // Destroy. Nothing in the graph clears memory (no memset nodes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -31,11 +33,9 @@ struct Batch {
   }
 };
 
-// Streams and events that exist only while the graph is captured.
+// The capture stream exists only while the graph is captured.
 struct CaptureScratch {
   hipStream_t cap = nullptr;
-  std::vector<hipStream_t> side;
-  std::vector<hipEvent_t> ev;
   bool capturing = false;
   ~CaptureScratch()
   {
@@ -43,8 +43,6 @@ struct CaptureScratch {
       hipGraph_t g = nullptr;
       if (hipStreamEndCapture(cap, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
     }
-    for (auto e : ev) (void)hipEventDestroy(e);
-    for (auto s : side) (void)hipStreamDestroy(s);
     if (cap) (void)hipStreamDestroy(cap);
   }
 };
@@ -118,24 +116,26 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
     delete b;
     return api_fail("batch validation launches failed");
   }
+  // Launch order: largest problem (sites x RHS x N^2) first.
+  std::vector<int> order(nprob);
+  for (int i = 0; i < nprob; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    const double wa = double(sites_per_parity(geoms[a])) * b->probs[a].nrhs * geoms[a].N * geoms[a].N;
+    const double wc = double(sites_per_parity(geoms[c])) * b->probs[c].nrhs * geoms[c].N * geoms[c].N;
+    return wa > wc;
+  });
   {
     CaptureScratch cs;
     bool ok = hipStreamCreateWithFlags(&cs.cap, hipStreamNonBlocking) == hipSuccess;
-    cs.side.assign(nprob, nullptr);
-    cs.ev.assign(nprob + 1, nullptr);
-    for (int i = 0; ok && i < nprob; ++i) ok = hipStreamCreateWithFlags(&cs.side[i], hipStreamNonBlocking) == hipSuccess;
-    for (int i = 0; ok && i <= nprob; ++i) ok = hipEventCreateWithFlags(&cs.ev[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
       delete b;
-      return api_fail("batch: stream or event creation failed");
+      return api_fail("batch: stream creation failed");
     }
     ok = hipStreamBeginCapture(cs.cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
     cs.capturing = ok;
-    ok = ok && hipEventRecord(cs.ev[nprob], cs.cap) == hipSuccess;
-    for (int i = 0; ok && i < nprob; ++i) {
-      ok = hipStreamWaitEvent(cs.side[i], cs.ev[nprob], 0) == hipSuccess &&
-           !launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.side[i]), &err) &&
-           hipEventRecord(cs.ev[i], cs.side[i]) == hipSuccess && hipStreamWaitEvent(cs.cap, cs.ev[i], 0) == hipSuccess;
+    for (int j = 0; ok && j < nprob; ++j) {
+      const int i = order[j];
+      ok = !launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.cap), &err);
     }
     hipGraph_t g = nullptr;
     if (cs.capturing) {

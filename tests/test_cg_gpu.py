@@ -29,7 +29,7 @@ def mhat(g, v, L, kappa, dagger):
     return v - kappa ** 2 * oracle.dslash_many(g, t, L, 0, dagger)
 
 
-def solve(lib, dev, g, b, L, N, kappa, sloppy, tol=1e-10, interval=50):
+def solve(lib, dev, g, b, L, N, kappa, sloppy, tol=1e-10, interval=50, delta=0.0, max_iter=5000):
     k = b.shape[0]
     geom = _lib.geom(L, N)
     gu = dev_tensor(g, dev)
@@ -38,7 +38,8 @@ def solve(lib, dev, g, b, L, N, kappa, sloppy, tol=1e-10, interval=50):
     bp = dev_tensor(fields.to_packed(b), dev)
     xp = torch.empty_like(bp)
     prm = _lib.CgParam()
-    prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 5000, interval, sloppy
+    prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, max_iter, interval, sloppy
+    prm.reliable_delta = delta
     res = (ctypes.c_double * k)()
     _lib.check(lib.qcuMrhsCgSolve(xp.data_ptr(), bp.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res,
                                   stream()))
@@ -116,3 +117,20 @@ def test_cg_inverter_full_system(lib, dev):
     print(f"cg_inverter full-system relative residual {rel:.2e}")
     assert rel < 1e-9
     assert x.shape == (2, S, 4, N)
+
+
+@pytest.mark.parametrize("delta", [3e-3, 1e-3])
+def test_cg_half_sparse_reliable_updates_converge(lib, dev, delta):
+    """FP16 sloppy CG with rare reliable updates (the iterated residual may
+    fall 1e3x between true residuals): the re-orthogonalised search direction
+    keeps it converging in about as many iterations as FP64 CG (before that
+    step such runs diverged)."""
+    L, N, k, kappa = (8, 8, 8, 16), 3, 8, 0.135
+    g = fields.gauge_field(L, N, seed=7)
+    b = fields.cg_sources(L, N, k, seed=8, n_point=4)
+    _, p64, _ = solve(lib, dev, g, b, L, N, kappa, _lib.DOUBLE)
+    x, p16, res = solve(lib, dev, g, b, L, N, kappa, _lib.HALF, delta=delta, max_iter=4 * p64.iterations)
+    print(f"delta={delta}: fp64 {p64.iterations} it, fp16 {p16.iterations} it / {p16.reliable_updates} updates")
+    assert max(res) <= 1e-10
+    assert p16.iterations <= 2 * p64.iterations
+    assert check_solution(g, b, x, L, kappa, 1e-10) <= 2e-10
