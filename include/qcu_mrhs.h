@@ -107,7 +107,9 @@ int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void
 
 /* ---- tuning ------------------------------------------------------------- */
 /* Times every instantiated tile on the given fields (out is overwritten),
- * records the fastest for (N, nrhs, prec, lattice) and returns it. */
+ * records the fastest for (N, nrhs, prec, lattice) and returns it. With
+ * QCU_MRHS_TUNE_CACHE=<file> in the environment, results persist in that text
+ * file (keyed by GPU architecture) and are reused by later processes. */
 int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *g,
                 qcuMrhsTile *best, double *best_us, void *stream);
 /* Tile the dispatcher uses for a shape (tuned if known, else the default). */

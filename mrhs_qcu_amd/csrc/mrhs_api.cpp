@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -65,12 +67,72 @@ int prec_bytes(int prec)
 long long sites_per_parity(const Geom &g) { return 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3]; }
 
 // ---------------------------------------------------------------------------
-// Tuning cache: (N, nrhs, prec, L, skip mask) -> tile.
+// Tuning cache: (N, nrhs, prec, L, skip mask) -> tile. With the environment
+// variable QCU_MRHS_TUNE_CACHE=<file>, entries for the current GPU
+// architecture are read from that text file on first use and every new
+// result is appended to it (one line per entry, later lines win):
+//   arch N nrhs prec Lx Ly Lz Lt mask R block lds t_block z_block hints occ traversal
 namespace {
 using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
+
+std::string device_arch()
+{
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+  std::string a = p.gcnArchName;
+  const size_t colon = a.find(':');
+  return colon == std::string::npos ? a : a.substr(0, colon);
+}
+
+const char *tune_file() { return getenv("QCU_MRHS_TUNE_CACHE"); }
+
+void load_tune_file(std::map<TuneKey, Tile> &m)
+{
+  const char *path = tune_file();
+  if (!path || !path[0]) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  const std::string arch = device_arch();
+  char a[64];
+  int v[17];
+  while (fscanf(f, "%63s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", a, &v[0], &v[1], &v[2], &v[3], &v[4],
+                &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13], &v[14], &v[15]) == 17) {
+    if (arch != a) continue;
+    Tile t;
+    t.R = v[8];
+    t.block = v[9];
+    t.lds = v[10];
+    t.tblock = v[11];
+    t.zblock = v[12];
+    t.hints = v[13];
+    t.occ = v[14];
+    t.ystream = v[15];
+    m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
+  }
+  fclose(f);
+}
+
+void append_tune_file(const TuneKey &k, const Tile &t)
+{
+  const char *path = tune_file();
+  if (!path || !path[0]) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), std::get<0>(k),
+          std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k),
+          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream);
+  fclose(f);
+}
+
 std::map<TuneKey, Tile> &tune_cache()
 {
   static std::map<TuneKey, Tile> m;
+  static bool loaded = false;
+  if (!loaded) {
+    loaded = true;
+    load_tune_file(m);
+  }
   return m;
 }
 std::mutex &tune_mutex()
@@ -119,6 +181,7 @@ void set_tile(const Geom &g, int prec, int k, const Tile &t)
 {
   std::lock_guard<std::mutex> lk(tune_mutex());
   tune_cache()[tune_key(g, prec, k)] = t;
+  append_tune_file(tune_key(g, prec, k), t);
 }
 
 std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)

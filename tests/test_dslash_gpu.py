@@ -139,6 +139,41 @@ def test_every_tile_agrees(lib, dev):
         assert n >= 96
 
 
+@pytest.mark.parametrize("prec", [_lib.HALF, _lib.BF16])
+def test_low_precision_every_tile(lib, dev, prec):
+    """FP16 (packed-half dot-product path) and BF16 storage: every
+    instantiated tile, N = 2 and 3, both parities and dagger signs, within the
+    stated storage tolerance of the FP64 oracle; all tiles agree with each
+    other to a few half-precision roundings."""
+    for N, L, k in ((3, (8, 4, 6, 8), 8), (2, (4, 4, 4, 8), 4)):
+        g = fields.gauge_field(L, N, seed=11 + N, smear_eps=0.3)
+        psi = fields.gaussian_spinors(L, N, k, seed=12 + N)
+        for parity, dagger in ((0, 0), (1, 1)):
+            want = oracle.dslash(g, psi, L, parity, dagger)
+            geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+            n = 0
+            first = None
+            for R, lds, block, (tb, zb), trav in itertools.product((1, 2, 4), (0, 1), (64, 256), ((1, 0), (2, 3)),
+                                                                    (0, 1)):
+                if k % R or L[3] % tb:
+                    continue
+                pout = torch.zeros_like(pin)
+                rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger,
+                                       geom, _lib.tile(R, block, lds, tb, zb, 0, 0, trav), stream())
+                if rc != 0:
+                    msg = lib.qcuMrhsLastError().decode()
+                    assert "no kernel instantiated" in msg or "LDS" in msg, msg
+                    continue
+                got = unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+                assert rel_err(got, want) < TOL[prec], (N, R, lds, block, tb, zb, trav)
+                if first is None:
+                    first = got
+                else:
+                    assert rel_err(got, first) < TOL[prec] / 4, (N, R, lds, block, tb, zb, trav)
+                n += 1
+            assert n >= 8, (N, n)
+
+
 @pytest.mark.parametrize("grid", [(2, 1, 1, 1), (1, 1, 1, 2), (2, 2, 2, 2)])
 def test_boundary_skip(lib, dev, grid):
     L, N, k = (4, 4, 4, 4), 3, 2
