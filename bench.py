@@ -220,6 +220,40 @@ def batch_benchmark(lib, _lib, torch, steps=50, warmup=5):
             "hbm_frac": tot_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_alg_per_replay": tot_bytes}
 
 
+def qcu_h_end_to_end(lib, _lib, torch, L=(24, 24, 24, 48), N=3, k=16, reps=20):
+    """The reference's call protocol through include/qcu.h, FP32 user buffers:
+    per call k pushBackFermions (input copied at push) + one startDslash
+    (synchronous). Wall time per call, host included."""
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    S = fields.half_volume(L)
+    grid, param = _lib.QcuGrid(), _lib.QcuParam()
+    for d in range(4):
+        grid.grid_size[d] = 1
+        param.lattice_size[d] = L[d]
+    lib.initGridSize(ctypes.byref(grid), ctypes.byref(param), N)
+    gu = fields.gauge_field_device(L, N, seed=3, device=dev)
+    lib.loadQcuGauge(gu.data_ptr())
+    lib.getDslash(0, 0.0, N, k, 1)
+    ins = [torch.randn(S * 4 * N * 2, device=dev, dtype=torch.float32) for _ in range(k)]
+    outs = [torch.empty_like(t) for t in ins]
+    torch.cuda.synchronize()
+    for i in range(3):
+        for r in range(k):
+            lib.pushBackFermions(outs[r].data_ptr(), ins[r].data_ptr())
+        lib.startDslash(i & 1, 0)
+    t0 = time.perf_counter()
+    for i in range(reps):
+        for r in range(k):
+            lib.pushBackFermions(outs[r].data_ptr(), ins[r].data_ptr())
+        lib.startDslash(i & 1, 0)
+    wall = (time.perf_counter() - t0) / reps
+    del gu
+    return {"workload": f"qcu.h push x{k} + startDslash, su{N} {'x'.join(map(str, L))} fp32 user layout",
+            "ms_per_call": wall * 1e3, "sites_rhs_per_s": S * k / wall}
+
+
 def cpu_baseline(L, N, k, budget_s=12.0):
     """FP32 C oracle (OpenMP) on the host: same workload, bounded repetitions."""
     import numpy as np
@@ -336,6 +370,7 @@ def main():
             "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tile": t3}
         extra["batch"] = batch_benchmark(lib, _lib, torch)
+        extra["qcu_h_end_to_end"] = qcu_h_end_to_end(lib, _lib, torch)
         line["extra"] = extra
         if not args.no_cg:
             line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))

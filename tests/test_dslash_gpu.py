@@ -155,7 +155,7 @@ def test_low_precision_every_tile(lib, dev, prec):
             first = None
             for R, lds, block, (tb, zb), trav in itertools.product((1, 2, 4), (0, 1), (64, 256), ((1, 0), (2, 3)),
                                                                     (0, 1)):
-                if k % R or L[3] % tb:
+                if k % R or L[3] % tb or (zb and L[2] % zb):
                     continue
                 pout = torch.zeros_like(pin)
                 rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger,
