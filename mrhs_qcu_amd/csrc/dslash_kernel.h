@@ -338,75 +338,115 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
 }
 
 // ---------------------------------------------------------------------------
-// binary16 storage path. Spinor and link components stay packed (re, im)
-// half pairs and every complex multiply-accumulate of the mat-vec is two
-// v_dot2_f32_f16 (d = s0.lo*s1.lo + s0.hi*s1.hi + s2, products and sum in
-// FP32):
+// 16-bit storage paths (binary16 and bfloat16). Spinor and link components
+// stay packed (re, im) pairs of 16-bit values and every complex
+// multiply-accumulate of the mat-vec is two dot2 instructions
+// (v_dot2_f32_f16 / v_dot2_f32_bf16: d = s0.lo*s1.lo + s0.hi*s1.hi + s2,
+// products and sum in FP32):
 //   Im(V h) = dot2(V, hs),  Re(V h) = dot2(Vc, hs),
 // with hs = (Im h, Re h) and Vc = (-Im V, Re V). The projection produces hs
-// directly (one packed-half FMA per colour, the half swap folded into its
-// operand selection), and Vc is formed once per link and shared by the R
-// right-hand sides of the lane.
+// directly and Vc is formed once per link, shared by the R right-hand sides
+// of the lane. binary16 projects with one packed-half FMA per colour (the
+// swap folded into its operand selection); bfloat16 has no packed arithmetic,
+// so it projects in FP32 and rounds the swapped pair back with one
+// v_cvt_pk_bf16_f32 (round to nearest even).
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ h2 as_h2(const c16 &v)
-{
-  h2 r;
-  r.x = v.x;
-  r.y = v.y;
-  return r;
-}
-__device__ __forceinline__ h2 swap_h2(h2 h) { return __builtin_shufflevector(h, h, 1, 0); }
+template <class C> struct Pair;
 
-// swap(base + (phase * s) * z) for the phase codes of add_phase; ss = (s, s).
-template <int KP>
-__device__ __forceinline__ h2 proj_swapped_h2(h2 base, h2 z, h2 ss)
-{
-  if constexpr (KP == 0) return __builtin_elementwise_fma(ss, swap_h2(z), swap_h2(base));
-  else if constexpr (KP == 1) return __builtin_elementwise_fma(-ss, swap_h2(z), swap_h2(base));
-  else if constexpr (KP == 2) {  // i s z = s (-Im z, Re z); swapped: (s Re z, -s Im z)
-    const h2 sg = {ss.x, _Float16(-ss.y)};
-    return __builtin_elementwise_fma(sg, z, swap_h2(base));
-  } else {  // -i s z = s (Im z, -Re z); swapped: (-s Re z, s Im z)
-    const h2 sg = {_Float16(-ss.x), ss.y};
-    return __builtin_elementwise_fma(sg, z, swap_h2(base));
+template <> struct Pair<c16> {
+  using V = h2;
+  __device__ __forceinline__ static V load(const c16 &c)
+  {
+    V r;
+    r.x = c.x;
+    r.y = c.y;
+    return r;
   }
-}
+  __device__ __forceinline__ static V swap(V h) { return __builtin_shufflevector(h, h, 1, 0); }
+  // swap(base + (phase * s) * z) for the phase codes of add_phase.
+  template <int KP>
+  __device__ __forceinline__ static V proj_swapped(V base, V z, float s)
+  {
+    const V ss = {_Float16(s), _Float16(s)};
+    if constexpr (KP == 0) return __builtin_elementwise_fma(ss, swap(z), swap(base));
+    else if constexpr (KP == 1) return __builtin_elementwise_fma(-ss, swap(z), swap(base));
+    else if constexpr (KP == 2) {  // i s z = s (-Im z, Re z); swapped: (s Re z, -s Im z)
+      const V sg = {ss.x, _Float16(-ss.y)};
+      return __builtin_elementwise_fma(sg, z, swap(base));
+    } else {  // -i s z = s (Im z, -Re z); swapped: (-s Re z, s Im z)
+      const V sg = {_Float16(-ss.x), ss.y};
+      return __builtin_elementwise_fma(sg, z, swap(base));
+    }
+  }
+  __device__ __forceinline__ static V link_c(V v)  // (-Im V, Re V)
+  {
+    const V m = {_Float16(-1.0f), _Float16(1.0f)};
+    return swap(v) * m;
+  }
+  __device__ __forceinline__ static float dot(V a, V b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+};
 
-// Vc = (-Im V, Re V).
-__device__ __forceinline__ h2 link_c_h2(h2 v)
-{
-  const h2 m = {_Float16(-1.0f), _Float16(1.0f)};
-  return swap_h2(v) * m;
-}
+template <> struct Pair<cb16> {
+  using V = bf2;
+  __device__ __forceinline__ static V load(const cb16 &c) { return __builtin_bit_cast(V, c); }
+  template <int KP>
+  __device__ __forceinline__ static V proj_swapped(V base, V z, float s)
+  {
+    const f2v b = __builtin_convertvector(base, f2v), w = __builtin_convertvector(z, f2v);
+    float hr, hi;
+    if constexpr (KP == 0) {
+      hr = fmaf(s, w.x, b.x);
+      hi = fmaf(s, w.y, b.y);
+    } else if constexpr (KP == 1) {
+      hr = fmaf(-s, w.x, b.x);
+      hi = fmaf(-s, w.y, b.y);
+    } else if constexpr (KP == 2) {
+      hr = fmaf(-s, w.y, b.x);
+      hi = fmaf(s, w.x, b.y);
+    } else {
+      hr = fmaf(s, w.y, b.x);
+      hi = fmaf(-s, w.x, b.y);
+    }
+    const f2v hs = {hi, hr};
+    return __builtin_convertvector(hs, V);
+  }
+  __device__ __forceinline__ static V link_c(V v)  // (-Im V, Re V): swap the halves, flip the low sign bit
+  {
+    const uint32_t u = __builtin_bit_cast(uint32_t, v);
+    return __builtin_bit_cast(V, ((u >> 16) | (u << 16)) ^ 0x8000u);
+  }
+  __device__ __forceinline__ static float dot(V a, V b, float c)
+  {
+    return __builtin_amdgcn_fdot2_f32_bf16(a, b, c, false);
+  }
+};
 
-// acc += hop in direction D for one RHS. Each mat-vec chain starts from -z
-// (z an SGPR zero): the negated scalar addend selects the three-address form
-// of the first v_dot2_f32_f16, so no register has to be zeroed or copied.
-template <int N, int D>
-__device__ __forceinline__ void hop_one_h2(cpx<float> (&acc)[4][N], const h2 (&V)[N][N], const h2 (&Vc)[N][N],
-                                           const h2 (&psi)[4][N], h2 ss, float s, float z)
+// acc += hop in direction D for one RHS (16-bit storage).
+template <class C, int N, int D>
+__device__ __forceinline__ void hop_one_p(cpx<float> (&acc)[4][N], const typename Pair<C>::V (&V)[N][N],
+                                          const typename Pair<C>::V (&Vc)[N][N],
+                                          const typename Pair<C>::V (&psi)[4][N], float s)
 {
+  using P = Pair<C>;
   constexpr HopEntry e = hop_entry(D);
-  h2 h0s[N], h1s[N];
+  typename P::V h0s[N], h1s[N];
 #pragma unroll
   for (int c = 0; c < N; ++c) {
-    h0s[c] = proj_swapped_h2<e.k0>(psi[0][c], psi[e.a0][c], ss);
-    h1s[c] = proj_swapped_h2<e.k1>(psi[1][c], psi[e.a1][c], ss);
+    h0s[c] = P::template proj_swapped<e.k0>(psi[0][c], psi[e.a0][c], s);
+    h1s[c] = P::template proj_swapped<e.k1>(psi[1][c], psi[e.a1][c], s);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    cpx<float> g0, g1;
-    g0.re = __builtin_amdgcn_fdot2(Vc[i][0], h0s[0], -z, false);
-    g0.im = __builtin_amdgcn_fdot2(V[i][0], h0s[0], -z, false);
-    g1.re = __builtin_amdgcn_fdot2(Vc[i][0], h1s[0], -z, false);
-    g1.im = __builtin_amdgcn_fdot2(V[i][0], h1s[0], -z, false);
+    cpx<float> g0{0.f, 0.f}, g1{0.f, 0.f};
 #pragma unroll
-    for (int j = 1; j < N; ++j) {
-      g0.re = __builtin_amdgcn_fdot2(Vc[i][j], h0s[j], g0.re, false);
-      g0.im = __builtin_amdgcn_fdot2(V[i][j], h0s[j], g0.im, false);
-      g1.re = __builtin_amdgcn_fdot2(Vc[i][j], h1s[j], g1.re, false);
-      g1.im = __builtin_amdgcn_fdot2(V[i][j], h1s[j], g1.im, false);
+    for (int j = 0; j < N; ++j) {
+      g0.re = P::dot(Vc[i][j], h0s[j], g0.re);
+      g0.im = P::dot(V[i][j], h0s[j], g0.im);
+      g1.re = P::dot(Vc[i][j], h1s[j], g1.re);
+      g1.im = P::dot(V[i][j], h1s[j], g1.im);
     }
     acc[0][i].re += g0.re;
     acc[0][i].im += g0.im;
@@ -417,51 +457,51 @@ __device__ __forceinline__ void hop_one_h2(cpx<float> (&acc)[4][N], const h2 (&V
   }
 }
 
-template <int N, int R, class A>
-__device__ __forceinline__ void load_spinor_h2(h2 (&psi)[R][4][N], const DslashArgs &a, int site, int slot)
+template <class C, int N, int R>
+__device__ __forceinline__ void load_spinor_p(typename Pair<C>::V (&psi)[R][4][N], const DslashArgs &a, int site,
+                                              int slot)
 {
-  const CVec<c16, R> *p = reinterpret_cast<const CVec<c16, R> *>(static_cast<const c16 *>(a.in) +
-                                                                  size_t(site) * 4 * N * a.k + size_t(slot) * R);
+  const CVec<C, R> *p = reinterpret_cast<const CVec<C, R> *>(static_cast<const C *>(a.in) +
+                                                              size_t(site) * 4 * N * a.k + size_t(slot) * R);
   const size_t stride = size_t(a.k) / R;
 #pragma unroll
   for (int sc = 0; sc < 4 * N; ++sc) {
-    CVec<c16, R> v = p[sc * stride];
+    CVec<C, R> v = p[sc * stride];
 #pragma unroll
-    for (int r = 0; r < R; ++r) psi[r][sc / N][sc % N] = as_h2(v.v[r]);
+    for (int r = 0; r < R; ++r) psi[r][sc / N][sc % N] = Pair<C>::load(v.v[r]);
   }
 }
 
-template <int N>
-__device__ __forceinline__ void load_link_h2(h2 (&V)[N][N], const c16 *blk8, int slot)
+template <class C, int N>
+__device__ __forceinline__ void load_link_p(typename Pair<C>::V (&V)[N][N], const C *blk8, int slot)
 {
-  const c16 *p = blk8 + slot * N * N;
+  const C *p = blk8 + slot * N * N;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) V[i][j] = as_h2(p[i * N + j]);
+    for (int j = 0; j < N; ++j) V[i][j] = Pair<C>::load(p[i * N + j]);
 }
 
-template <int D, int N, int R>
-__device__ __forceinline__ void sweep_h2(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
-                                         const c16 *blk8, h2 ss, float s, h2 (&psi_cur)[R][4][N],
-                                         h2 (&V_cur)[N][N], h2 (&psi_nxt)[R][4][N], h2 (&V_nxt)[N][N])
+template <class C, int D, int N, int R>
+__device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
+                                        const C *blk8, float s, typename Pair<C>::V (&psi_cur)[R][4][N],
+                                        typename Pair<C>::V (&V_cur)[N][N], typename Pair<C>::V (&psi_nxt)[R][4][N],
+                                        typename Pair<C>::V (&V_nxt)[N][N])
 {
   if constexpr (D + 1 < 8) {
-    load_spinor_h2<N, R, float>(psi_nxt, a, g.nbr[D + 1], slot);
-    load_link_h2<N>(V_nxt, blk8, native_link_slot(D + 1));
+    load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[D + 1], slot);
+    load_link_p<C, N>(V_nxt, blk8, native_link_slot(D + 1));
   }
   if (g.live[D]) {
-    h2 Vc[N][N];
+    typename Pair<C>::V Vc[N][N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) Vc[i][j] = link_c_h2(V_cur[i][j]);
-    float z;
-    asm("s_mov_b32 %0, 0" : "=s"(z));
+      for (int j = 0; j < N; ++j) Vc[i][j] = Pair<C>::link_c(V_cur[i][j]);
 #pragma unroll
-    for (int r = 0; r < R; ++r) hop_one_h2<N, D>(acc[r], V_cur, Vc, psi_cur[r], ss, s, z);
+    for (int r = 0; r < R; ++r) hop_one_p<C, N, D>(acc[r], V_cur, Vc, psi_cur[r], s);
   }
-  if constexpr (D + 1 < 8) sweep_h2<D + 1, N, R>(acc, a, g, slot, blk8, ss, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  if constexpr (D + 1 < 8) sweep_p<C, D + 1, N, R>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
 }
 
 // Checkerboard index of a logical site number.
@@ -555,13 +595,13 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};  // x + (-0) == x: folds
 
-    if constexpr (SL == SL_PACKED && GL == GL_NATIVE && std::is_same<CS, c16>::value &&
-                  std::is_same<CG, c16>::value) {
-      h2 psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
-      const h2 ss = {_Float16(a.sign), _Float16(a.sign)};
-      load_spinor_h2<N, R, float>(psi0, a, g.nbr[0], slot);
-      load_link_h2<N>(V0, blk8, native_link_slot(0));
-      sweep_h2<0, N, R>(acc, a, g, slot, blk8, ss, float(s), psi0, V0, psi1, V1);
+    if constexpr (SL == SL_PACKED && GL == GL_NATIVE && std::is_same<CS, CG>::value &&
+                  (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) && std::is_same<A, float>::value) {
+      using PV = typename Pair<CS>::V;
+      PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
+      load_spinor_p<CS, N, R>(psi0, a, g.nbr[0], slot);
+      load_link_p<CS, N>(V0, blk8, native_link_slot(0));
+      sweep_p<CS, 0, N, R>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
