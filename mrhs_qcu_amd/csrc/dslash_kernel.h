@@ -174,7 +174,18 @@ struct SiteGeo {
   int idx;       // checkerboard index ((t*Lz+z)*Ly+y)*subLx+xh
   int nbr[8];    // neighbour index per direction D
   int live[8];   // 0 if the term is skipped at a local boundary
+  int xh;        // checkerboard x index
+  // X- neighbour sharing: the X- neighbour of a site is the X+ neighbour of
+  // the site before it in its x row (same y, z, t). When that site's lanes sit
+  // in the same wavefront (lane - xstride), the X- spinor is taken from them
+  // with a cross-lane read instead of a second load; xown marks the lanes
+  // that load it themselves (first site of a row or of a wavefront).
+  int xown, xsrc;
 };
+
+// 32-bit cross-lane read of lane `src` (ds_bpermute); doubles take two.
+__device__ __forceinline__ float lane_read(float v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ double lane_read(double v, int src) { return __shfl(v, src, 64); }
 
 // Logical site number -> coordinates for the traversal orders
 //   (t_out, z_out, t_in, z_in, y, x/2)   blocks of t_block x z_block slices, or
@@ -236,6 +247,7 @@ __device__ __forceinline__ SiteGeo site_geometry(uint32_t sl, const DslashArgs &
   g.nbr[5] = idx + (z == 0 ? (a.Lz - 1) * sz : -sz);
   g.nbr[6] = idx + (int(t) + 1 == a.Lt ? -(a.Lt - 1) * st : st);
   g.nbr[7] = idx + (t == 0 ? (a.Lt - 1) * st : -st);
+  g.xh = int(xh);
   g.live[0] = !(a.skip[0] && int(xh) == a.subLx - 1 && p != eo);
   g.live[1] = !(a.skip[0] && xh == 0 && p == eo);
   g.live[2] = !(a.skip[1] && int(y) == a.Ly - 1);
@@ -326,7 +338,19 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
                                       const CG *blk8, A s, cpx<A> (&psi_cur)[R][4][N], cpx<A> (&V_cur)[N][N],
                                       cpx<A> (&psi_nxt)[R][4][N], cpx<A> (&V_nxt)[N][N])
 {
-  if constexpr (D + 1 < 8) {
+  if constexpr (D == 0) {
+    // X- data: every lane reads its predecessor's X+ spinor, then the lanes
+    // without a predecessor in the row overwrite it with their own load.
+    fetch_link<1, N, GL, CG, A>(V_nxt, a, g, blk8);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+          psi_nxt[r][sp][c] = cpx<A>{lane_read(psi_cur[r][sp][c].re, g.xsrc), lane_read(psi_cur[r][sp][c].im, g.xsrc)};
+    if (g.xown) load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[1], slot);
+  } else if constexpr (D + 1 < 8) {
     load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[D + 1], slot);
     fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
   }
@@ -488,7 +512,18 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
                                         typename Pair<C>::V (&V_cur)[N][N], typename Pair<C>::V (&psi_nxt)[R][4][N],
                                         typename Pair<C>::V (&V_nxt)[N][N])
 {
-  if constexpr (D + 1 < 8) {
+  if constexpr (D == 0) {
+    load_link_p<C, N>(V_nxt, blk8, native_link_slot(1));
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+          psi_nxt[r][sp][c] = __builtin_bit_cast(
+              typename Pair<C>::V, lane_read(__builtin_bit_cast(float, psi_cur[r][sp][c]), g.xsrc));
+    if (g.xown) load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[1], slot);
+  } else if constexpr (D + 1 < 8) {
     load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[D + 1], slot);
     load_link_p<C, N>(V_nxt, blk8, native_link_slot(D + 1));
   }
@@ -501,6 +536,7 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
 #pragma unroll
     for (int r = 0; r < R; ++r) hop_one_p<C, N, D>(acc[r], V_cur, Vc, psi_cur[r], s);
   }
+
   if constexpr (D + 1 < 8) sweep_p<C, D + 1, N, R>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
 }
 
@@ -582,7 +618,16 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
   for (int r = 0; r < R; ++r) nrm[r] = A(0);
   if (active) {
-    const SiteGeo g = site_geometry(sl, a);
+    SiteGeo g = site_geometry(sl, a);
+    {
+      // Lane gid - xstride holds the same RHS slot of logical site sl - 1,
+      // which is x index xh - 1 of the same row (x is the fastest coordinate
+      // of every traversal order).
+      const int xstride = SITE_MINOR ? 1 : int(a.P);
+      const int lw = int(threadIdx.x & 63u);
+      g.xown = (lw < xstride || g.xh == 0) ? 1 : 0;
+      g.xsrc = g.xown ? lw : lw - xstride;
+    }
     if constexpr (GL == GL_NATIVE && !LDS)
       blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * N * N;
 
