@@ -22,10 +22,10 @@ long long blas_num_blocks(long long n, int k);
 // p = gamma r + beta p (gamma = NULL: 1) ; ps = p / sc_p ; partial |p|^2
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
                          const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st);
-// partial Re <a, b>
-hipError_t blas_dot_re(int prec, const void *a, const void *b, double *partial, long long n, int k, hipStream_t st);
 hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
-hipError_t blas_residual(void *r, const void *b, const void *Ax, double *partial, long long n, int k, hipStream_t st);
+// r = b - Ax (b, Ax FP64; r stored in prec_k), partials |r|^2 and Re <r, p>.
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, const void *p, double *partial_rr,
+                         double *partial_rp, long long n, int k, hipStream_t st);
 hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st);
 hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
 hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st);

@@ -59,23 +59,6 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
   block_partial(acc, rhs, k, partial);
 }
 
-// partial Re <a, b> = sum Re(conj(a) b)
-template <class C>
-__global__ void __launch_bounds__(kBlasBlock) dot_re_kernel(const C *__restrict__ a, const C *__restrict__ b,
-                                                            double *partial, long long n, int k)
-{
-  using A = typename Acc<C>::T;
-  const long long T = (long long)gridDim.x * blockDim.x;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = int(g % k);
-  double acc = 0.0;
-  for (long long e = g; e < n; e += T) {
-    const cpx<A> x = ld_cvt<A>(a[e]), y = ld_cvt<A>(b[e]);
-    acc += double(x.re) * y.re + double(x.im) * y.im;
-  }
-  block_partial(acc, rhs, k, partial);
-}
-
 // partial |v|^2
 template <class C>
 __global__ void __launch_bounds__(kBlasBlock) norm2_kernel(const C *__restrict__ v, double *partial, long long n,
@@ -94,21 +77,31 @@ __global__ void __launch_bounds__(kBlasBlock) norm2_kernel(const C *__restrict__
 }
 
 // r = b - Ax (FP64) ; partial |r|^2
-__global__ void __launch_bounds__(kBlasBlock) residual_kernel(c64 *__restrict__ r, const c64 *__restrict__ b,
-                                                              const c64 *__restrict__ Ax, double *partial,
-                                                              long long n, int k)
+// r = b - Ax computed in FP64 and stored in the Krylov precision CK;
+// partials |r|^2 (of the FP64 values) and Re <r, p> (of the stored r).
+template <class CK>
+__global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *__restrict__ r, const c64 *__restrict__ b,
+                                                              const c64 *__restrict__ Ax, const CK *__restrict__ p,
+                                                              double *partial_rr, double *partial_rp, long long n,
+                                                              int k)
 {
+  using A = typename Acc<CK>::T;
   const long long T = (long long)gridDim.x * blockDim.x;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = int(g % k);
-  double acc = 0.0;
+  double acc = 0.0, dot = 0.0;
   for (long long e = g; e < n; e += T) {
-    c64 bb = b[e], aa = Ax[e];
-    c64 rr{bb.x - aa.x, bb.y - aa.y};
-    r[e] = rr;
-    acc += rr.x * rr.x + rr.y * rr.y;
+    const c64 bb = b[e], aa = Ax[e];
+    const cpx<double> rr{bb.x - aa.x, bb.y - aa.y};
+    const CK rs = StCvt<CK>::go(rr);
+    r[e] = rs;
+    acc += rr.re * rr.re + rr.im * rr.im;
+    const cpx<A> rv = ld_cvt<A>(rs), pv = ld_cvt<A>(p[e]);
+    dot += double(rv.re) * pv.re + double(rv.im) * pv.im;
   }
-  block_partial(acc, rhs, k, partial);
+  block_partial(acc, rhs, k, partial_rr);
+  __syncthreads();
+  block_partial(dot, rhs, k, partial_rp);
 }
 
 // xh += x (x converted to FP64), then x = 0
@@ -299,23 +292,15 @@ hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int
   return hipGetLastError();
 }
 
-hipError_t blas_dot_re(int prec, const void *a, const void *b, double *partial, long long n, int k, hipStream_t st)
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, const void *p, double *partial_rr,
+                         double *partial_rp, long long n, int k, hipStream_t st)
 {
-  if (prec == PREC_DOUBLE)
-    hipLaunchKernelGGL(dot_re_kernel<c64>, QM_GRID(n, k), static_cast<const c64 *>(a), static_cast<const c64 *>(b),
-                       partial, n, k);
-  else if (prec == PREC_SINGLE)
-    hipLaunchKernelGGL(dot_re_kernel<c32>, QM_GRID(n, k), static_cast<const c32 *>(a), static_cast<const c32 *>(b),
-                       partial, n, k);
+  if (prec_k == PREC_DOUBLE)
+    hipLaunchKernelGGL(residual_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
+                       static_cast<const c64 *>(Ax), static_cast<const c64 *>(p), partial_rr, partial_rp, n, k);
   else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
-hipError_t blas_residual(void *r, const void *b, const void *Ax, double *partial, long long n, int k, hipStream_t st)
-{
-  hipLaunchKernelGGL(residual_kernel, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
-                     static_cast<const c64 *>(Ax), partial, n, k);
+    hipLaunchKernelGGL(residual_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
+                       static_cast<const c64 *>(Ax), static_cast<const c32 *>(p), partial_rr, partial_rp, n, k);
   return hipGetLastError();
 }
 

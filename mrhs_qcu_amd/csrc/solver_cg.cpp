@@ -343,15 +343,12 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     if (mixed && !hip_ok(blas_accumulate_zero(pk, x_out, xk, n, k, st), err, "accumulate")) return false;
     if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
     if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
-    if (!hip_ok(blas_residual(rh, bhat, ax64, s.partial, n, k, st), err, "residual")) return false;
+    // r = b^ - M^+M^ x_h straight into the Krylov vector, with |r|^2 and
+    // <r, p> (for re-orthogonalising p) from the same pass.
+    if (!hip_ok(blas_residual(pk, r, bhat, ax64, p, s.partial, partial_p, n, k, st), err, "residual")) return false;
     if (!hip_ok(blas_reduce(s.partial, nb_blas, k, rr_true, st), err, "reduce")) return false;
+    if (!hip_ok(blas_reduce(partial_p, nb_blas, k, cs.rdotp, st), err, "reduce")) return false;
     if (!hip_ok(cg_check_true(cs, rr_true, k, stop2, st), err, "check")) return false;
-    if (mixed) {
-      std::string e2;
-      if (convert_complex(r, pk, rh, PREC_DOUBLE, n, st, &e2)) return *err = e2, false;
-    } else if (!hip_ok(hipMemcpyAsync(r, rh, b64, hipMemcpyDeviceToDevice, st), err, "copy")) {
-      return false;
-    }
     ++updates;
     since_update = 0;
     if (!hip_ok(hipMemcpyAsync(done.data(), cs.done, k * sizeof(int), hipMemcpyDeviceToHost, st), err, "d2h"))
@@ -414,8 +411,6 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       if (converged || iter == max_iter) break;
       // beta = |r_true|^2 / |r_old|^2 ; p = gamma r + beta p with the old p
       // made orthogonal to r_true.
-      QM_HIP(blas_dot_re(pk, r, p, s.partial, n, k, st));
-      QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.rdotp, st));
       QM_HIP(cg_beta(cs, k, stop2, st));
       QM_HIP(cg_orth(cs, k, st));
       QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, sc_p, partial_p, n, k, st));
