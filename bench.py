@@ -10,7 +10,8 @@ The JSON line also carries
                 over the kernel's average launch time (HIP events on the
                 launch stream) against 8 TB/s; `traffic` is the rocprofv3
                 FETCH_SIZE + WRITE_SIZE per launch (scaled by a copy-kernel
-                calibration) when a measured summary is committed under profiles/;
+                calibration) when a measured summary is committed under profiles/,
+                with the tile it was measured on (`traffic_tile`);
   cpu_baseline: the FP32 C oracle (OpenMP) on the host cores, same workload,
                 a bounded number of applications;
   extra:        FP16 / BF16 / FP64 storage at the same size and the CG
@@ -285,15 +286,17 @@ def cpu_baseline(L, N, k, budget_s=12.0):
 
 
 def traffic_from_profile(name):
+    """(bytes per launch, tile it was measured with) from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary, or (None, None)."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
-        return None
+        return None, None
     try:
         with open(p) as f:
             d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["hbm_bytes_per_launch"]), d.get("tile")
     except Exception:
-        return None
+        return None, None
 
 
 def main():
@@ -332,6 +335,7 @@ def main():
     ba = bytes_alg(S, k, N, pb, pb, pb)
     achieved = ba / (ms * 1e-3) / 1e9
     wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_{args.precision}"
+    traffic, traffic_tile = traffic_from_profile(f"{wl}_traffic.json")
     line = {
         "metric": "MRHS dslash sites\u00b7RHS/s + %HBM roofline; mixed vs FP64 CG time to 1e-10",
         "value": value,
@@ -351,7 +355,8 @@ def main():
                                      "occupancy", "traversal"), tile)),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(f"{wl}_traffic.json"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_tile": traffic_tile,
                      "bytes_alg_per_launch": ba},
     }
     if not args.profile and not args.no_extra:
