@@ -310,6 +310,33 @@ def test_qcu_h_call_sequence(lib, dev):
             assert rel_err(got, oracle.dslash(g, psi, L, parity, 0)) < TOL[prec]
 
 
+def test_qcu_h_many_rhs_chunks(lib, dev):
+    """startDslash with more queued right-hand sides than one pointer pack
+    holds (40 > 32: two launches), N = 2 and N = 4, against the oracle; the
+    queue is empty afterwards and a second round works."""
+    for N, L, k in ((2, (4, 4, 4, 4), 40), (4, (4, 4, 4, 8), 35)):
+        g = fields.gauge_field(L, N, seed=30 + N)
+        psi = fields.gaussian_spinors(L, N, k, seed=31 + N)
+        grid = _lib.QcuGrid()
+        param = _lib.QcuParam()
+        for d in range(4):
+            grid.grid_size[d] = 1
+            param.lattice_size[d] = L[d]
+        lib.initGridSize(ctypes.byref(grid), ctypes.byref(param), N)
+        gd = dev_tensor(g, dev)
+        lib.loadQcuGauge(gd.data_ptr())
+        lib.getDslash(0, 0.0, N, k, 2)
+        ins = [dev_tensor(psi[r], dev) for r in range(k)]
+        for parity, dagger in ((1, 0), (0, 1)):
+            outs = [torch.full_like(t, float("nan")) for t in ins]
+            torch.cuda.synchronize()
+            for r in range(k):
+                lib.pushBackFermions(outs[r].data_ptr(), ins[r].data_ptr())
+            lib.startDslash(parity, dagger)
+            got = np.stack([host_complex(t) for t in outs])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger)) < TOL[2], (N, parity, dagger)
+
+
 @pytest.mark.parametrize("prec", [_lib.SINGLE, _lib.HALF])
 def test_full_size_properties(lib, dev, prec):
     """24^3x48, nrhs = 16: RHS independence (a permutation of the RHS permutes
