@@ -66,13 +66,34 @@ int registry_size()
 
 Tile default_tile(int N, int k, int prec)
 {
+  // The shape the autotuner picks most often on gfx950 for N = 3: two RHS per
+  // lane (four for 16-bit storage), links staged in LDS, 128-lane workgroups.
   Tile t;
-  t.block = 256;
+  t.block = 128;
   t.tblock = 1;
   t.R = 1;
   t.lds = 0;
-  if (k % 2 == 0 && N <= 5 && prec != PREC_DOUBLE) t.R = 2;
-  if (k / t.R >= 4 && N <= 5) t.lds = 1;
+  const bool half = prec == PREC_HALF || prec == PREC_BF16;
+  if (half && k % 4 == 0 && N <= 3) t.R = 4;
+  else if (k % 2 == 0 && N <= 5) t.R = 2;
+  if (k / t.R >= 2 && N <= 5) t.lds = 1;
+  return t;
+}
+
+Tile default_tile(const Geom &g, int k, int prec)
+{
+  Tile t = default_tile(g.N, k, prec);
+  // y-streamed blocks of the largest t and z extents that divide the lattice.
+  int tb = 0, zb = 0;
+  for (int c : {6, 4, 3, 2})
+    if (!tb && g.L[3] % c == 0 && c < g.L[3]) tb = c;
+  for (int c : {4, 6, 3, 2})
+    if (!zb && g.L[2] % c == 0 && c < g.L[2]) zb = c;
+  if (tb && zb) {
+    t.tblock = tb;
+    t.zblock = zb;
+    t.ystream = 1;
+  }
   return t;
 }
 

@@ -102,5 +102,6 @@ int launch_dslash(const DslashCall &c, std::string *err);
 long dslash_num_blocks(const DslashCall &c);
 // Default tile for a packed launch (used when the autotuner has no entry).
 Tile default_tile(int N, int k, int prec);
+Tile default_tile(const Geom &g, int k, int prec);  // with a traversal for the lattice
 
 }  // namespace qmrhs

@@ -155,7 +155,7 @@ Tile tile_for(const Geom &g, int prec, int k)
     auto it = tune_cache().find(tune_key(g, prec, k));
     if (it != tune_cache().end()) return it->second;
   }
-  Tile t = default_tile(g.N, k, prec);
+  Tile t = default_tile(g, k, prec);
   // Fall back to a plain tile when the preferred one is not instantiated.
   KernelKey key{SL_PACKED, GL_NATIVE, g.N, t.R, t.lds, 0, prec, prec, prec};
   if (k % t.R || !find_kernel(key)) {
