@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
     ap.add_argument("--cg-only", action="store_true", help="only the CG comparison (printed as one JSON line)")
     ap.add_argument("--cg-L", type=int, nargs=4, default=[32, 32, 32, 64])
+    ap.add_argument("--cg-solver", choices=["both", "fp64", "mixed"], default="both",
+                    help="with --cg-only: run one solver only (profiling)")
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
     ap.add_argument("--tile", type=str, default=None,
                     help="fixed tile R,block,lds,tblock[,zblock,hints,occupancy,traversal] for the headline kernel "
@@ -106,7 +108,8 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     return ms, best.as_tuple()
 
 
-def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0):
+def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0,
+                 solvers=("fp64", "mixed_fp16")):
     """Even-odd preconditioned MRHS CG on M^+M^ to a true relative residual
     tol: pure FP64 vs mixed precision (FP16 dslash storage, FP32 Krylov
     vectors, FP64 reductions and reliable updates), same GPU, same inputs.
@@ -139,6 +142,8 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
            "sources": f"{npt} point sources + {k - npt} Gaussian"}
     sols = {}
     for name, sloppy in (("fp64", _lib.DOUBLE), ("mixed_fp16", _lib.HALF)):
+        if name not in solvers:
+            continue
         x = torch.empty_like(b)
         prm = _lib.CgParam()
         res = (ctypes.c_double * k)()
@@ -156,6 +161,8 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
         out[name] = {"seconds": wall, "iterations": prm.iterations, "reliable_updates": prm.reliable_updates,
                      "max_true_rel_residual": max(res), "converged": bool(max(res) <= tol)}
         sols[name] = x
+    if len(sols) < 2:
+        return out
     d = (sols["mixed_fp16"] - sols["fp64"]).permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
     n = sols["fp64"].permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
     out["max_rel_solution_diff"] = float((d / n).max())
@@ -325,7 +332,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
         return
     if args.cg_only:
-        print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))), flush=True)
+        solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",)}[args.cg_solver]
+        print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), solvers=solvers)), flush=True)
         return
     pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE}[args.precision]
     pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8}[args.precision]
