@@ -76,7 +76,7 @@ Tile default_tile(int N, int k, int prec)
   const bool half = prec == PREC_HALF || prec == PREC_BF16;
   if (half && k % 4 == 0 && N <= 3) t.R = 4;
   else if (k % 2 == 0 && N <= 5) t.R = 2;
-  if (k / t.R >= 2 && N <= 5) t.lds = 1;
+  if (k / t.R >= 2) t.lds = 1;
   return t;
 }
 

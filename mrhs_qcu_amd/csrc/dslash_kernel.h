@@ -333,7 +333,9 @@ __device__ __forceinline__ void fetch_link(cpx<A> (&V)[N][N], const DslashArgs &
 // so each wave keeps one direction of loads in flight while it computes.
 // Every neighbour index is in range (periodic wrap), so the prefetch is
 // unconditional; a skipped boundary term only skips the arithmetic.
-template <int D, int N, int R, int SL, int GL, class CS, class CG, class A>
+// LL: links are staged in LDS; they are then read right before use (LDS
+// latency is short) instead of being prefetched into a second register set.
+template <int D, int N, int R, int SL, int GL, class CS, class CG, class A, bool LL>
 __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
                                       const CG *blk8, A s, cpx<A> (&psi_cur)[R][4][N], cpx<A> (&V_cur)[N][N],
                                       cpx<A> (&psi_nxt)[R][4][N], cpx<A> (&V_nxt)[N][N])
@@ -341,7 +343,7 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
   if constexpr (D == 0) {
     // X- data: every lane reads its predecessor's X+ spinor, then the lanes
     // without a predecessor in the row overwrite it with their own load.
-    fetch_link<1, N, GL, CG, A>(V_nxt, a, g, blk8);
+    if constexpr (!LL) fetch_link<1, N, GL, CG, A>(V_nxt, a, g, blk8);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -352,13 +354,19 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
     if (g.xown) load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[1], slot);
   } else if constexpr (D + 1 < 8) {
     load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[D + 1], slot);
-    fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
+    if constexpr (!LL) fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
   }
   if (g.live[D]) {
+    if constexpr (LL) fetch_link<D, N, GL, CG, A>(V_cur, a, g, blk8);
 #pragma unroll
     for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V_cur, psi_cur[r], s);
   }
-  if constexpr (D + 1 < 8) sweep<D + 1, N, R, SL, GL, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  if constexpr (D + 1 < 8) {
+    if constexpr (LL)
+      sweep<D + 1, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_cur, psi_cur, V_nxt);
+    else
+      sweep<D + 1, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -506,14 +514,14 @@ __device__ __forceinline__ void load_link_p(typename Pair<C>::V (&V)[N][N], cons
     for (int j = 0; j < N; ++j) V[i][j] = Pair<C>::load(p[i * N + j]);
 }
 
-template <class C, int D, int N, int R>
+template <class C, int D, int N, int R, bool LL>
 __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
                                         const C *blk8, float s, typename Pair<C>::V (&psi_cur)[R][4][N],
                                         typename Pair<C>::V (&V_cur)[N][N], typename Pair<C>::V (&psi_nxt)[R][4][N],
                                         typename Pair<C>::V (&V_nxt)[N][N])
 {
   if constexpr (D == 0) {
-    load_link_p<C, N>(V_nxt, blk8, native_link_slot(1));
+    if constexpr (!LL) load_link_p<C, N>(V_nxt, blk8, native_link_slot(1));
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -525,9 +533,10 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
     if (g.xown) load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[1], slot);
   } else if constexpr (D + 1 < 8) {
     load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[D + 1], slot);
-    load_link_p<C, N>(V_nxt, blk8, native_link_slot(D + 1));
+    if constexpr (!LL) load_link_p<C, N>(V_nxt, blk8, native_link_slot(D + 1));
   }
   if (g.live[D]) {
+    if constexpr (LL) load_link_p<C, N>(V_cur, blk8, native_link_slot(D));
     typename Pair<C>::V Vc[N][N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -537,7 +546,12 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
     for (int r = 0; r < R; ++r) hop_one_p<C, N, D>(acc[r], V_cur, Vc, psi_cur[r], s);
   }
 
-  if constexpr (D + 1 < 8) sweep_p<C, D + 1, N, R>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  if constexpr (D + 1 < 8) {
+    if constexpr (LL)
+      sweep_p<C, D + 1, N, R, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_cur, psi_cur, V_nxt);
+    else
+      sweep_p<C, D + 1, N, R, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  }
 }
 
 // Checkerboard index of a logical site number.
@@ -645,13 +659,14 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       using PV = typename Pair<CS>::V;
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor_p<CS, N, R>(psi0, a, g.nbr[0], slot);
-      load_link_p<CS, N>(V0, blk8, native_link_slot(0));
-      sweep_p<CS, 0, N, R>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
+      if constexpr (!LDS) load_link_p<CS, N>(V0, blk8, native_link_slot(0));
+      sweep_p<CS, 0, N, R, LDS>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
-      fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
-      sweep<0, N, R, SL, GL, CS, CG, A>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
+      constexpr bool LL = LDS && GL == GL_NATIVE;
+      if constexpr (!LL) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
+      sweep<0, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
     }
 
     // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.

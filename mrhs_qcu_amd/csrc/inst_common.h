@@ -18,8 +18,9 @@
   QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CG, CO, A) \
   QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CG, CO, A)
 
-#define QM_PACKED_BASIC(N, CS, CG, CO, A) \
-  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CG, CO, A)
+#define QM_PACKED_BASIC(N, CS, CG, CO, A)                                          \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CG, CO, A) \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CG, CO, A)
 
 // Pointer layouts (one RHS per lane).
 #define QM_PTRS(N, CS, A)                                                             \
@@ -35,5 +36,6 @@
 #define QM_MIXED_R4(N, CS)                                                                             \
   QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float) \
   QM_REG(N, 4, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CS, ::qmrhs::c32, float)
-#define QM_MIXED_BASIC(N, CS) \
-  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float)
+#define QM_MIXED_BASIC(N, CS)                                                                          \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, false, false, CS, CS, ::qmrhs::c32, float) \
+  QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CS, ::qmrhs::c32, float)
