@@ -235,6 +235,10 @@ int launch_dslash(const DslashCall &c, std::string *err)
   KernelKey key{c.sl, c.gl, c.geom.N, 1, 0, sm ? 1 : 0, c.prec_s, c.prec_g, c.prec_o};
   LaunchFn fn = find_kernel(key);
   if (!fn) return fail(err, "no kernel instantiated for this layout / precision / N"), 1;
+  // AoS pointers: links staged in LDS when that variant exists and fits.
+  KernelKey key_lds = key;
+  key_lds.lds = 1;
+  LaunchFn fn_lds = c.sl == SL_AOS_PTRS ? find_kernel(key_lds) : nullptr;
   for (int r0 = 0; r0 < c.k; r0 += kMaxPtrs) {
     const int kk = (c.k - r0) < kMaxPtrs ? (c.k - r0) : kMaxPtrs;
     DslashArgs b = a;
@@ -256,7 +260,19 @@ int launch_dslash(const DslashCall &c, std::string *err)
     if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
     b.nlanes = uint32_t(lanes);
     b.nblocks = uint32_t((lanes + B - 1) / B);
-    hipError_t e = fn(b, dim3(b.nblocks), dim3(B), 0, c.stream);
+    uintptr_t ptr_bits = 0;
+    for (int r = 0; r < kk; ++r) ptr_bits |= uintptr_t(b.out_ptrs.p[r]) | uintptr_t(b.in_ptrs.p[r]);
+    b.aos_vec = (ptr_bits % 16) == 0 ? 1 : 0;
+    size_t shm = 0;
+    LaunchFn f = fn;
+    if (fn_lds) {
+      const size_t lds_bytes = size_t((B - 1) / kk + 2) * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
+      if (lds_bytes <= size_t(kMaxDynLds)) {
+        f = fn_lds;
+        shm = lds_bytes;
+      }
+    }
+    hipError_t e = f(b, dim3(b.nblocks), dim3(B), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
   }
   return 0;

@@ -74,6 +74,7 @@ struct DslashArgs {
   uint32_t nblocks;
   FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S, div_Ly;
   int ystream;  // traversal (t_out, z_out, y, t_in, z_in, x/2) instead of (t_out, z_out, t_in, z_in, y, x/2)
+  int aos_vec;  // pointer layouts: every buffer 16-byte aligned (16-byte loads and stores)
 };
 
 // Hop table: direction D = 2*mu + (backward); projection partner spins a0/a1
@@ -277,6 +278,18 @@ __device__ __forceinline__ void load_spinor(cpx<A> (&psi)[R][4][N], const Dslash
   } else if constexpr (SL == SL_AOS_PTRS) {
     static_assert(R == 1, "pointer layouts hold one RHS per lane");
     const CS *p = static_cast<const CS *>(a.in_ptrs.p[slot]) + size_t(site) * 4 * N;
+    constexpr int C = 16 / int(sizeof(CS));  // complex numbers per 16-byte load
+    if constexpr (C > 1) {
+      if (a.aos_vec) {  // every buffer 16-byte aligned: the site's 4N components in 16-byte pieces
+#pragma unroll
+        for (int q = 0; q < 4 * N / C; ++q) {
+          const CVec<CS, C> v = reinterpret_cast<const CVec<CS, C> *>(p)[q];
+#pragma unroll
+          for (int j = 0; j < C; ++j) psi[0][(q * C + j) / N][(q * C + j) % N] = ld_cvt<A>(v.v[j]);
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int sc = 0; sc < 4 * N; ++sc) psi[0][sc / N][sc % N] = ld_cvt<A>(p[sc]);
   } else {
@@ -750,22 +763,46 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
         else po[sc * stride] = o;
       }
     } else {
-      CO *po;
-      size_t es;
-      if constexpr (SL == SL_AOS_PTRS) {
-        po = static_cast<CO *>(a.out_ptrs.p[slot]) + size_t(g.idx) * 4 * N;
-        es = 1;
-      } else {
-        po = static_cast<CO *>(a.out_ptrs.p[slot]) + g.idx;
-        es = size_t(a.S);
-      }
+      constexpr int C = 16 / int(sizeof(CO));
+      bool vec_done = false;
+      if constexpr (SL == SL_AOS_PTRS && C > 1) {
+        if (a.aos_vec) {
+          CVec<CO, C> *pv = reinterpret_cast<CVec<CO, C> *>(static_cast<CO *>(a.out_ptrs.p[slot]) +
+                                                            size_t(g.idx) * 4 * N);
 #pragma unroll
-      for (int sc = 0; sc < 4 * N; ++sc) {
-        cpx<A> v = acc[0][sc / N][sc % N];
-        v.re *= fin[0];
-        v.im *= fin[0];
-        nrm[0] += v.re * v.re + v.im * v.im;
-        po[sc * es] = StCvt<CO>::go(v);
+          for (int q = 0; q < 4 * N / C; ++q) {
+            CVec<CO, C> o;
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+              cpx<A> v = acc[0][(q * C + j) / N][(q * C + j) % N];
+              v.re *= fin[0];
+              v.im *= fin[0];
+              nrm[0] += v.re * v.re + v.im * v.im;
+              o.v[j] = StCvt<CO>::go(v);
+            }
+            pv[q] = o;
+          }
+          vec_done = true;
+        }
+      }
+      if (!vec_done) {
+        CO *po;
+        size_t es;
+        if constexpr (SL == SL_AOS_PTRS) {
+          po = static_cast<CO *>(a.out_ptrs.p[slot]) + size_t(g.idx) * 4 * N;
+          es = 1;
+        } else {
+          po = static_cast<CO *>(a.out_ptrs.p[slot]) + g.idx;
+          es = size_t(a.S);
+        }
+#pragma unroll
+        for (int sc = 0; sc < 4 * N; ++sc) {
+          cpx<A> v = acc[0][sc / N][sc % N];
+          v.re *= fin[0];
+          v.im *= fin[0];
+          nrm[0] += v.re * v.re + v.im * v.im;
+          po[sc * es] = StCvt<CO>::go(v);
+        }
       }
     }
   }

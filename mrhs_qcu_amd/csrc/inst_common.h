@@ -25,6 +25,7 @@
 // Pointer layouts (one RHS per lane).
 #define QM_PTRS(N, CS, A)                                                             \
   QM_REG(N, 1, ::qmrhs::SL_AOS_PTRS, ::qmrhs::GL_NATIVE, false, false, CS, CS, CS, A) \
+  QM_REG(N, 1, ::qmrhs::SL_AOS_PTRS, ::qmrhs::GL_NATIVE, true, false, CS, CS, CS, A)  \
   QM_REG(N, 1, ::qmrhs::SL_SOA_PTRS, ::qmrhs::GL_REF_SOA, false, true, CS, CS, CS, A)
 
 // Mixed storage for the solver: sloppy input, FP32 output.
