@@ -121,3 +121,28 @@ def test_fastdiv_exact(tmp_path):
                    capture_output=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.startswith("OK"), r.stdout
+
+
+def test_default_tiles_are_launchable_shapes(lib):
+    """The tile the dispatcher uses without tuning (qcuMrhsGetTile, no GPU
+    needed) always satisfies the launch rules: R divides nrhs, the block is a
+    multiple of 64 up to 256, t_block / z_block divide Lt / Lz, and LDS link
+    staging fits the per-workgroup budget."""
+    from mrhs_qcu_amd import _lib
+
+    lds_budget = 65536 - 2048
+    bytes_per_complex = {_lib.HALF: 4, _lib.BF16: 4, _lib.SINGLE: 8, _lib.DOUBLE: 16}
+    for L in ((4, 4, 4, 4), (8, 8, 8, 8), (24, 24, 24, 48), (16, 16, 16, 32), (12, 12, 12, 24), (6, 4, 10, 14)):
+        for N in range(2, 9):
+            for k in (1, 2, 3, 6, 12, 16, 24, 40):
+                for prec in (_lib.HALF, _lib.BF16, _lib.SINGLE, _lib.DOUBLE):
+                    t = _lib.Tile()
+                    assert lib.qcuMrhsGetTile(_lib.geom(L, N), prec, k, ctypes.byref(t)) == 0
+                    assert k % t.rhs_per_lane == 0, (L, N, k, prec, t.as_tuple())
+                    assert t.block % 64 == 0 and 64 <= t.block <= 256
+                    assert t.t_block >= 1 and L[3] % t.t_block == 0
+                    assert t.z_block == 0 or L[2] % t.z_block == 0
+                    if t.link_lds:
+                        P = k // t.rhs_per_lane
+                        shm = ((t.block - 1) // P + 2) * 8 * N * N * bytes_per_complex[prec]
+                        assert shm <= lds_budget, (L, N, k, prec, t.as_tuple())
