@@ -128,8 +128,6 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.xb = c.xb;
   a.partial = c.partial;
   a.rx_r = c.rx_r;
-  a.rx_x = c.rx_x;
-  a.rx_p = c.rx_p;
   a.rx_alpha = c.rx_alpha;
   a.sign = c.dagger ? -1.0f : 1.0f;
   a.parity = c.parity;
@@ -191,9 +189,8 @@ int launch_dslash(const DslashCall &c, std::string *err)
     if (c.rx_r) {
       if (c.prec_o != PREC_SINGLE && c.prec_o != PREC_DOUBLE)
         return fail(err, "the fused CG update needs FP32 or FP64 output"), 1;
-      if (!c.rx_x || !c.rx_p || !c.rx_alpha) return fail(err, "fused CG update: null r, x, p or alpha"), 1;
-      if ((uintptr_t(c.rx_r) | uintptr_t(c.rx_x) | uintptr_t(c.rx_p)) % al)
-        return fail(err, "packed fields must be 16-byte aligned"), 1;
+      if (!c.rx_alpha) return fail(err, "fused CG update: null alpha"), 1;
+      if (uintptr_t(c.rx_r) % al) return fail(err, "packed fields must be 16-byte aligned"), 1;
     }
     if (c.gl != GL_NATIVE) return fail(err, "packed spinors use the native gauge layout"), 1;
     const int lds = c.tile.lds ? 1 : 0;

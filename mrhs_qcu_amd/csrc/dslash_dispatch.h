@@ -89,9 +89,8 @@ struct DslashCall {
   const float *sc_in = nullptr, *sc_x = nullptr, *sc_out = nullptr;
   double xa = 0.0, xb = 1.0;
   double *partial = nullptr;  // [nblocks][k]; nblocks from dslash_num_blocks()
-  // Fused CG update r -= alpha out, x += alpha p (FP32 packed output only).
-  void *rx_r = nullptr, *rx_x = nullptr;
-  const void *rx_p = nullptr;
+  // Fused CG update r -= alpha out (FP32 or FP64 packed output only).
+  void *rx_r = nullptr;
   const double *rx_alpha = nullptr;
   hipStream_t stream = nullptr;
 };

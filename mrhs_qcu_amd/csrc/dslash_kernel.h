@@ -52,11 +52,9 @@ struct DslashArgs {
   double xa, xb;          // true out = xa * x + xb * D(in)   (x null: xb * D(in))
   double *partial;        // per-workgroup per-RHS sum of |true out|^2, [nblocks][k] (nullable)
   // Fused CG update (packed layout, FP32 or FP64 output; rx_r null: off):
-  // with v = the true out, r -= alpha v and x += alpha p in place (packed
-  // fields of the output type), out is not written and partial receives
-  // |r|^2 instead.
-  void *rx_r, *rx_x;
-  const void *rx_p;
+  // with v = the true out, r -= alpha v in place (a packed field of the
+  // output type), out is not written and partial receives |r|^2 instead.
+  void *rx_r;
   const double *rx_alpha;
   float sign;             // +1 (daggerFlag 0) or -1 (daggerFlag 1)
   int k;                  // RHS count (RHS stride of the packed layout)
@@ -699,8 +697,6 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
         const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
         const size_t stride = size_t(a.k) / R;
         CVec<CO, R> *pr = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_r) + base);
-        CVec<CO, R> *pxk = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_x) + base);
-        const CVec<CO, R> *pp = reinterpret_cast<const CVec<CO, R> *>(static_cast<const CO *>(a.rx_p) + base);
         const CVec<CS, R> *px =
             a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
         A al[R];
@@ -710,8 +706,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
         for (int sc = 0; sc < 4 * N; ++sc) {
           CVec<CS, R> xv;
           if (px) xv = px[sc * stride];
-          CVec<CO, R> rv = pr[sc * stride], xk = pxk[sc * stride];
-          const CVec<CO, R> pv = pp[sc * stride];
+          CVec<CO, R> rv = pr[sc * stride];
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             cpx<A> v = acc[r][sc / N][sc % N];
@@ -724,12 +719,9 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
             }
             rv.v[r].x -= al[r] * v.re;
             rv.v[r].y -= al[r] * v.im;
-            xk.v[r].x += al[r] * pv.v[r].x;
-            xk.v[r].y += al[r] * pv.v[r].y;
             nrm[r] += rv.v[r].x * rv.v[r].x + rv.v[r].y * rv.v[r].y;
           }
           pr[sc * stride] = rv;
-          pxk[sc * stride] = xk;
         }
       }
     }

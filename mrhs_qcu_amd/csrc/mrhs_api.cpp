@@ -217,7 +217,7 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
           }
         }
         // Blocks streamed along y (t_block x z_block cross-sections).
-        for (int tb : {2, 3, 4, 6}) {
+        for (int tb : {2, 3, 4, 6, 8}) {
           if (g.L[3] % tb || tb >= g.L[3]) continue;
           for (int zb : {2, 3, 4, 6, 8}) {
             if (g.L[2] % zb || zb >= g.L[2]) continue;

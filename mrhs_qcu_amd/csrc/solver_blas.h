@@ -19,14 +19,19 @@ struct CgState {
 // partial-sum buffers hold blas_num_blocks(n, k) * k doubles).
 long long blas_num_blocks(long long n, int k);
 
-// p = gamma r + beta p (gamma = NULL: 1) ; ps = p / sc_p ; partial |p|^2
+// x += alpha p (x = NULL: skipped) ; p = gamma r + beta p (gamma = NULL: 1) ;
+// ps = p / sc_p ; partial |p|^2
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st);
+                         const double *gamma, void *x, const double *alpha, const float *sc_p, double *partial,
+                         long long n, int k, hipStream_t st);
 hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
 // r = b - Ax (b, Ax FP64; r stored in prec_k), partials |r|^2 and Re <r, p>.
 hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, const void *p, double *partial_rr,
                          double *partial_rp, long long n, int k, hipStream_t st);
-hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st);
+// xh (FP64) += x + alpha p, then x = 0 (x = NULL: xh += alpha p; p = NULL: no
+// alpha p term).
+hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, const double *alpha, long long n, int k,
+                           hipStream_t st);
 hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
 hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st);
 hipError_t cg_alpha(const CgState &s, int k, hipStream_t st);

@@ -30,13 +30,14 @@ __device__ __forceinline__ void block_partial(double v, int rhs, int k, double *
   for (int i = threadIdx.x; i < k; i += blockDim.x) partial[size_t(blockIdx.x) * k + i] = red[i];
 }
 
-// p = gamma r + beta p (gamma null: 1) ; p_s = p / sc_p (sloppy copy, may be
-// null) ; partial |p|^2
+// x += alpha p (x null: skipped) ; p = gamma r + beta p (gamma null: 1) ;
+// p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2
 template <class CK, class CS>
 __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p, CS *__restrict__ ps,
                                                               const CK *__restrict__ r,
                                                               const double *__restrict__ beta,
                                                               const double *__restrict__ gamma,
+                                                              CK *__restrict__ x, const double *__restrict__ alpha,
                                                               const float *__restrict__ sc_p, double *partial,
                                                               long long n, int k)
 {
@@ -46,10 +47,17 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
   const int rhs = int(g % k);
   const A b = A(beta[rhs]);
   const A c = gamma ? A(gamma[rhs]) : A(1);
+  const A al = x ? A(alpha[rhs]) : A(0);
   const float inv = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
   double acc = 0.0;
   for (long long e = g; e < n; e += T) {
     cpx<A> pp = ld_cvt<A>(p[e]), rr = ld_cvt<A>(r[e]);
+    if (x) {
+      cpx<A> xx = ld_cvt<A>(x[e]);
+      xx.re += al * pp.re;
+      xx.im += al * pp.im;
+      x[e] = StCvt<CK>::go(xx);
+    }
     pp.re = c * rr.re + b * pp.re;
     pp.im = c * rr.im + b * pp.im;
     p[e] = StCvt<CK>::go(pp);
@@ -104,19 +112,37 @@ __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *__restrict__ r
   block_partial(dot, rhs, k, partial_rp);
 }
 
-// xh += x (x converted to FP64), then x = 0
+// xh += x + alpha p (x, p nullable; the sum formed in x's precision, then
+// converted to FP64), then x = 0
 template <class CK>
-__global__ void __launch_bounds__(kBlasBlock) accumulate_zero_kernel(c64 *__restrict__ xh, CK *__restrict__ x,
-                                                                     long long n)
+__global__ void __launch_bounds__(kBlasBlock) accumulate_kernel(c64 *__restrict__ xh, CK *__restrict__ x,
+                                                                const CK *__restrict__ p,
+                                                                const double *__restrict__ alpha, long long n, int k)
 {
   const long long T = (long long)gridDim.x * blockDim.x;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += T) {
-    cpx<double> v = ld_cvt<double>(x[e]);
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  using A = typename Acc<CK>::T;
+  const A al = p ? A(alpha[int(g % k)]) : A(0);
+  for (long long e = g; e < n; e += T) {
+    cpx<double> v{0.0, 0.0};
+    if (x) {
+      // The step lands in x_k's precision first, as it would in x_k.
+      cpx<A> xx = ld_cvt<A>(x[e]);
+      if (p) {
+        const cpx<A> pp = ld_cvt<A>(p[e]);
+        xx.re += al * pp.re;
+        xx.im += al * pp.im;
+      }
+      v = {double(xx.re), double(xx.im)};
+      x[e] = StCvt<CK>::go(cpx<double>{0.0, 0.0});
+    } else if (p) {
+      const cpx<A> pp = ld_cvt<A>(p[e]);
+      v = {double(al * pp.re), double(al * pp.im)};
+    }
     c64 h = xh[e];
     h.x += v.re;
     h.y += v.im;
     xh[e] = h;
-    x[e] = StCvt<CK>::go(cpx<double>{0.0, 0.0});
   }
 }
 
@@ -251,24 +277,26 @@ long long blas_num_blocks(long long n, int k) { return grid_for(n, k); }
 #define QM_GRID(n, k) dim3(grid_for(n, k)), dim3(kBlasBlock), 0, st
 
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const double *gamma, const float *sc_p, double *partial, long long n, int k, hipStream_t st)
+                         const double *gamma, void *x, const double *alpha, const float *sc_p, double *partial,
+                         long long n, int k, hipStream_t st)
 {
   if (prec_k == PREC_DOUBLE) {
     hipLaunchKernelGGL((update_p_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(p), static_cast<c64 *>(ps),
-                       static_cast<const c64 *>(r), beta, gamma, sc_p, partial, n, k);
+                       static_cast<const c64 *>(r), beta, gamma, static_cast<c64 *>(x), alpha, sc_p, partial, n, k);
   } else {
     switch (prec_s) {
       case PREC_HALF:
         hipLaunchKernelGGL((update_p_kernel<c32, c16>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c16 *>(ps),
-                           static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
+                           static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x), alpha, sc_p, partial, n, k);
         break;
       case PREC_BF16:
         hipLaunchKernelGGL((update_p_kernel<c32, cb16>), QM_GRID(n, k), static_cast<c32 *>(p),
-                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
+                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x),
+                           alpha, sc_p, partial, n, k);
         break;
       default:
         hipLaunchKernelGGL((update_p_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c32 *>(ps),
-                           static_cast<const c32 *>(r), beta, gamma, sc_p, partial, n, k);
+                           static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x), alpha, sc_p, partial, n, k);
     }
   }
   return hipGetLastError();
@@ -304,12 +332,15 @@ hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, con
   return hipGetLastError();
 }
 
-hipError_t blas_accumulate_zero(int prec_k, void *xh, void *x, long long n, int k, hipStream_t st)
+hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, const double *alpha, long long n, int k,
+                           hipStream_t st)
 {
   if (prec_k == PREC_DOUBLE)
-    hipLaunchKernelGGL(accumulate_zero_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c64 *>(x), n);
+    hipLaunchKernelGGL(accumulate_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c64 *>(x),
+                       static_cast<const c64 *>(p), alpha, n, k);
   else
-    hipLaunchKernelGGL(accumulate_zero_kernel<c32>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c32 *>(x), n);
+    hipLaunchKernelGGL(accumulate_kernel<c32>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c32 *>(x),
+                       static_cast<const c32 *>(p), alpha, n, k);
   return hipGetLastError();
 }
 

@@ -119,8 +119,6 @@ struct Solver {
     DslashCall c;
     if (fuse) {
       c.rx_r = fuse->rx_r;
-      c.rx_x = fuse->rx_x;
-      c.rx_p = fuse->rx_p;
       c.rx_alpha = fuse->rx_alpha;
     }
     c.sl = SL_PACKED;
@@ -306,7 +304,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     QM_HIP(hipMemcpyAsync(r, bhat, b64, hipMemcpyDeviceToDevice, st));
   }
   QM_HIP(blas_zero(pk, p, n, k, st));
-  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, sc_p, partial_p, n, k, st));
+  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, nullptr, nullptr, sc_p, partial_p, n,
+                       k, st));
 
   std::vector<int> done(k, 0);
   std::vector<double> rrt(k, 0.0), b2h(k, 0.0), rr_ru(k, 0.0);
@@ -337,10 +336,14 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   QM_HIP(hipHostMalloc(&pin.p, size_t(k) * sizeof(double), hipHostMallocDefault));
   double *h_rr = static_cast<double *>(pin.p);
 
-  // True residual: x_h += x_k (mixed), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces
-  // the iterated norm, done flags; the Krylov residual restarts from r_h.
-  auto reliable_update = [&]() -> bool {
-    if (mixed && !hip_ok(blas_accumulate_zero(pk, x_out, xk, n, k, st), err, "accumulate")) return false;
+  // True residual: x_h += x_k (mixed) + alpha p (the step of the iteration
+  // just finished, when `step`), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces the
+  // iterated norm, done flags; the Krylov residual restarts from r_h.
+  auto reliable_update = [&](bool step) -> bool {
+    if ((mixed || step) && !hip_ok(blas_accumulate(pk, x_out, mixed ? xk : nullptr, step ? p : nullptr, cs.alpha,
+                                                   n, k, st),
+                                   err, "accumulate"))
+      return false;
     if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
     if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
     // r = b^ - M^+M^ x_h straight into the Krylov vector, with |r|^2 and
@@ -374,11 +377,11 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   converged = true;
   for (int i = 0; i < k; ++i) converged = converged && done[i];
 
-  // The r / x update is fused into the last operator launch of an iteration.
+  // r -= alpha Ap is fused into the last operator launch of an iteration;
+  // x += alpha p into the next p update (which reads the old p anyway), or
+  // into the accumulation of a reliable update.
   DslashCall fuse;
   fuse.rx_r = r;
-  fuse.rx_x = xk;
-  fuse.rx_p = p;
   fuse.rx_alpha = cs.alpha;
   while (!converged && iter < max_iter) {
     // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
@@ -390,7 +393,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
     // Ap = q - kappa^2 D^+_eo t2 stays in registers: the same launch does
-    // r -= alpha Ap, x += alpha p and the per-RHS |r|^2 partials (out unused).
+    // r -= alpha Ap and the per-RHS |r|^2 partials (out unused).
     QM_TRY(s.dslash(r, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
     QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.rr_new, st));
     QM_HIP(hipMemcpyAsync(h_rr, cs.rr_new, k * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -407,20 +410,24 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     }
     if (verbose && iter % 10 == 0) fprintf(stderr, "cg iter %d: iterated relative residual max %.3e\n", iter, mx);
     if (all_conv || drop || (mixed && since_update >= interval) || iter == max_iter) {
-      QM_TRY(reliable_update());
+      QM_TRY(reliable_update(true));
       if (converged || iter == max_iter) break;
       // beta = |r_true|^2 / |r_old|^2 ; p = gamma r + beta p with the old p
       // made orthogonal to r_true.
       QM_HIP(cg_beta(cs, k, stop2, st));
       QM_HIP(cg_orth(cs, k, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, sc_p, partial_p, n, k, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, nullptr, nullptr, sc_p, partial_p,
+                           n, k, st));
     } else {
-      // beta = |r|^2 / |r_old|^2 ; p = r + beta p
+      // x += alpha p ; beta = |r|^2 / |r_old|^2 ; p = r + beta p
       QM_HIP(cg_beta(cs, k, stop2, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, sc_p, partial_p, n, k, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, xk, cs.alpha, sc_p, partial_p, n,
+                           k, st));
     }
   }
-  if (!converged && since_update > 0) QM_TRY(reliable_update());
+  // Every iteration ends in a reliable update or in a p update that applied
+  // its x step, so x is complete here.
+  if (!converged && since_update > 0) QM_TRY(reliable_update(false));
   QM_HIP(hipMemcpyAsync(rrt.data(), rr_true, k * sizeof(double), hipMemcpyDeviceToHost, st));
   QM_HIP(hipMemcpyAsync(b2h.data(), cs.b2, k * sizeof(double), hipMemcpyDeviceToHost, st));
   QM_HIP(hipStreamSynchronize(st));
