@@ -105,6 +105,29 @@ int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const v
 int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void *gauge_soa, int prec, int nrhs,
                                  int parity, int dagger, const qcuMrhsGeom *g);
 
+/* ---- face exchange (one dimension split over processes) ------------------
+ * No counterpart in the reference, whose multi-process layer is declared but
+ * not implemented (include/qcuDesc/qcu_desc.h:53-97, boundary streams at
+ * include/qcu_kernel_param.cuh:11). With g->grid[mu] > 1, qcuMrhsDslash skips
+ * the terms that cross the local boundary of mu (the reference's semantics,
+ * wilson_dslash_4d.cuh:60,87,...); these two calls supply them:
+ *   qcuMrhsPackFaces  (on `in`, parity 1 - parity): to_bwd[mu] = projected
+ *     half spinors of the x_mu = 0 face, for the process behind in mu;
+ *     to_fwd[mu] = U^+ times the projected half spinors of the x_mu = L_mu - 1
+ *     face, for the process ahead;
+ *   qcuMrhsDslashFaces (on `out`, parity `parity`): adds the boundary terms,
+ *     from_fwd[mu] = the to_bwd[mu] of the process ahead, from_bwd[mu] = the
+ *     to_fwd[mu] of the process behind.
+ * Arrays of 4 device pointers; entries of unsplit dimensions are ignored.
+ * Face buffers hold qcuMrhsFaceBytes(g, nrhs, prec, mu) bytes, [F][2][N][nrhs]
+ * in prec (the precision of the spinors and the native gauge). Moving the
+ * buffers between processes (RCCL send/recv, peer copies) is the caller's. */
+size_t qcuMrhsFaceBytes(const qcuMrhsGeom *g, int nrhs, int prec, int mu);
+int qcuMrhsPackFaces(void *const *to_bwd, void *const *to_fwd, const void *in, const void *gauge, int prec, int nrhs,
+                     int parity, int dagger, const qcuMrhsGeom *g, void *stream);
+int qcuMrhsDslashFaces(void *out, const void *const *from_bwd, const void *const *from_fwd, const void *gauge,
+                       int prec, int nrhs, int parity, int dagger, const qcuMrhsGeom *g, void *stream);
+
 /* ---- tuning ------------------------------------------------------------- */
 /* Times every instantiated tile on the given fields (out is overwritten),
  * records the fastest for (N, nrhs, prec, lattice) and returns it. With

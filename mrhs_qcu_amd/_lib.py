@@ -54,7 +54,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay", "qcuMrhsDslashEx",
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
-                      "qcuMrhsBatchDestroy"]
+                      "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces"]
 
 _lib = None
 
@@ -99,6 +99,9 @@ def _declare(lib):
         "qcuMrhsBatchCreate": (I, [ctypes.POINTER(Problem), I, ctypes.POINTER(P), P]),
         "qcuMrhsBatchRun": (I, [P, P]),
         "qcuMrhsBatchDestroy": (I, [P]),
+        "qcuMrhsFaceBytes": (ctypes.c_size_t, [G, I, I, I]),
+        "qcuMrhsPackFaces": (I, [PP, PP, P, P, I, I, I, I, G, P]),
+        "qcuMrhsDslashFaces": (I, [P, PP, PP, P, I, I, I, I, G, P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])

@@ -74,6 +74,33 @@ def test_argument_validation_without_gpu(lib):
     assert "null batch handle" in lib.qcuMrhsLastError().decode()
 
 
+def test_face_sizes_and_validation(lib):
+    """Face buffers: F sites of one parity x 2 spins x N colours x nrhs; a
+    split dimension needs both face pointers."""
+    from mrhs_qcu_amd import _lib
+
+    L, N, k = (8, 6, 4, 10), 3, 4
+    g = _lib.geom(L, N, grid=(2, 1, 1, 2))
+    S = 4 * 6 * 4 * 10
+    per = 2 * N * k * 8
+    assert lib.qcuMrhsFaceBytes(g, k, _lib.SINGLE, 0) == (6 * 4 * 10 // 2) * per
+    assert lib.qcuMrhsFaceBytes(g, k, _lib.SINGLE, 1) == S // 6 * per
+    assert lib.qcuMrhsFaceBytes(g, k, _lib.SINGLE, 3) == S // 10 * per
+    assert lib.qcuMrhsFaceBytes(g, k, _lib.SINGLE, 4) == 0
+    # Every face holds half the sites of a slice: F = V / (2 L_mu) for each mu.
+    for mu in range(4):
+        assert lib.qcuMrhsFaceBytes(g, k, _lib.DOUBLE, mu) == 8 * 6 * 4 * 10 // (2 * L[mu]) * 2 * N * k * 16
+    # Dimension 0 present, dimension 3 missing: rejected before any launch.
+    four, keep = _lib.ptr_array([16, 0, 0, 0])
+    rc = lib.qcuMrhsPackFaces(four, four, 16, 16, _lib.SINGLE, k, 0, 0, g, None)
+    assert rc != 0 and "split dimension 3" in lib.qcuMrhsLastError().decode()
+    null4, keep2 = _lib.ptr_array([0, 0, 0, 0])
+    rc = lib.qcuMrhsPackFaces(null4, null4, 16, 16, _lib.SINGLE, k, 0, 0, g, None)
+    assert rc != 0 and "split dimension 0" in lib.qcuMrhsLastError().decode()
+    rc = lib.qcuMrhsDslashFaces(16, null4, null4, 16, _lib.SINGLE, k, 2, 0, g, None)
+    assert rc != 0 and "parity" in lib.qcuMrhsLastError().decode()
+
+
 DEATH = [
     ("lib.startDslash(0, 0)", "Qcu is not initialized."),
     ("lib.pushBackFermions(1, 1)", "Qcu is not initialized."),
