@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--tile", type=str, default=None,
                     help="fixed tile R,block,lds,tblock[,zblock,hints,occupancy,traversal] for the headline kernel "
                          "(skips the tuner)")
+    ap.add_argument("--faces-only", action="store_true", help="only the split-lattice face exchange timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
     ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
@@ -106,6 +107,40 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     return ms, best.as_tuple()
+
+
+def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0):
+    """The same workload on two-row SU(3) links (qcuMrhsDslash12, the tile of
+    the native layout); returns ms_per_step."""
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    S = fields.half_volume(L)
+    key = (tuple(L), 3, seed)
+    if key not in _GAUGE_CACHE:
+        _GAUGE_CACHE.clear()
+        _GAUGE_CACHE[key] = torch.view_as_real(fields.gauge_field_device(L, 3, seed=seed, device=dev)).reshape(-1)
+    gu = _GAUGE_CACHE[key]
+    dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
+    g12 = torch.empty(lib.qcuMrhsGauge12Bytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
+    # The generator folds antiperiodic time into the links: t_sign = -1.
+    _lib.check(lib.qcuMrhsPackGauge12(g12.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, -1, sp))
+    a = torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt)
+    bufs = [a, torch.empty_like(a)]
+    for i in range(warmup):
+        _lib.check(lib.qcuMrhsDslash12(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k,
+                                       i & 1, 0, -1, geom, None, sp))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for i in range(steps):
+        lib.qcuMrhsDslash12(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0,
+                            -1, geom, None, sp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
 
 
 def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0,
@@ -171,6 +206,66 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
 
 
 BATCH_WORKLOAD = [((16, 16, 16, 16), 2, 8), ((24, 24, 24, 48), 3, 12), ((16, 16, 16, 32), 4, 24), ((12, 12, 12, 24), 5, 6)]
+
+
+def faces_benchmark(lib, _lib, torch, L=(24, 24, 24, 24), grid=(1, 1, 1, 2), N=3, k=16, steps=200, warmup=20):
+    """One process's share of 24^3x48 split in two along t (24^3x24, FP32,
+    16 RHS): interior operator (boundary terms skipped), face pack and the
+    boundary terms from received faces (here the process's own faces), each
+    timed with HIP events on the launch stream. Face bytes: pack reads the
+    spinors of both faces and the links of one and writes two half-spinor
+    faces; the boundary launch reads the faces and one face's links and
+    reads and writes the output on both faces."""
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    geom = _lib.geom(L, N, grid=grid)
+    S = fields.half_volume(L)
+    prec = _lib.SINGLE
+    gu = torch.view_as_real(fields.gauge_field_device(L, N, seed=0, device=dev)).reshape(-1)
+    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, prec) // 4, dtype=torch.float32, device=dev)
+    _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
+    del gu
+    a = torch.randn(S * 4 * N * k * 2, device=dev, dtype=torch.float32)
+    b = torch.empty_like(a)
+    _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, None, None, sp))
+    tb, tf = [0] * 4, [0] * 4
+    keep = []
+    pack_bytes = faces_bytes = 0
+    for mu in range(4):
+        if grid[mu] > 1:
+            nb = lib.qcuMrhsFaceBytes(geom, k, prec, mu)
+            keep += [torch.empty(nb // 4, dtype=torch.float32, device=dev) for _ in range(2)]
+            tb[mu], tf[mu] = keep[-2].data_ptr(), keep[-1].data_ptr()
+            F = nb // (2 * N * k * 8)
+            spinor, half, link = 4 * N * k * 8, 2 * N * k * 8, N * N * 8
+            pack_bytes += F * (2 * spinor + link + 2 * half)
+            faces_bytes += F * (2 * half + link + 2 * 2 * spinor)
+    pb, kb = _lib.ptr_array(tb)
+    pf, kf = _lib.ptr_array(tf)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_int = timed(lambda: lib.qcuMrhsDslash(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, 0, 0, geom, None, sp))
+    ms_pack = timed(lambda: lib.qcuMrhsPackFaces(pb, pf, a.data_ptr(), gn.data_ptr(), prec, k, 0, 0, geom, sp))
+    ms_faces = timed(lambda: lib.qcuMrhsDslashFaces(b.data_ptr(), pf, pb, gn.data_ptr(), prec, k, 0, 0, geom, sp))
+    return {"workload": f"su3_eo_wilson_dslash_{'x'.join(map(str, L))}_grid{''.join(map(str, grid))}_nrhs{k}_fp32",
+            "interior_ms": ms_int, "pack_faces_ms": ms_pack, "dslash_faces_ms": ms_faces,
+            "face_bytes_per_direction": lib.qcuMrhsFaceBytes(geom, k, prec, 3),
+            "pack_GBs": pack_bytes / (ms_pack * 1e-3) / 1e9,
+            "faces_GBs": faces_bytes / (ms_faces * 1e-3) / 1e9}
 
 
 def batch_benchmark(lib, _lib, torch, steps=50, warmup=5):
@@ -331,6 +426,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
         return
+    if args.faces_only:
+        print(json.dumps(faces_benchmark(lib, _lib, torch)), flush=True)
+        return
     if args.cg_only:
         solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",)}[args.cg_solver]
         print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), solvers=solvers)), flush=True)
@@ -374,6 +472,13 @@ def main():
             bb = bytes_alg(S, k, N, b, b, b)
             extra[name] = {"ms_per_step": ms2, "sites_rhs_per_s": S * k / (ms2 * 1e-3),
                            "hbm_frac": bb / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tile": t2}
+        for name, prec, b in (("two_row_links_fp32", _lib.SINGLE, 4), ("two_row_links_fp16", _lib.HALF, 2)):
+            ms12 = time_dslash12(lib, _lib, torch, geom, prec, k, max(20, args.steps // 4), 10, L)
+            bb12 = S * (k * 8 * N * 2 * b + 8 * 2 * N * 2 * b)
+            extra[name] = {"ms_per_step": ms12, "sites_rhs_per_s": S * k / (ms12 * 1e-3),
+                           "hbm_frac": bb12 / (ms12 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                           "hbm_frac_vs_full_link_bytes": bytes_alg(S, k, N, b, b, b) / (ms12 * 1e-3) / 1e9 /
+                           HBM_PEAK_GBS}
         Lc2, kc2 = (16, 16, 16, 32), 12
         g2 = _lib.geom(Lc2, N)
         ms3, t3 = time_dslash(lib, _lib, torch, g2, _lib.SINGLE, kc2, max(50, args.steps // 2), 10, Lc2, N)
@@ -383,6 +488,7 @@ def main():
             "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tile": t3}
         extra["batch"] = batch_benchmark(lib, _lib, torch)
+        extra["faces"] = faces_benchmark(lib, _lib, torch)
         extra["qcu_h_end_to_end"] = qcu_h_end_to_end(lib, _lib, torch)
         line["extra"] = extra
         if not args.no_cg:

@@ -105,6 +105,26 @@ int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const v
 int qcuMrhsDslashReferenceLayout(void *const *out_soa, void *const *in_soa, void *gauge_soa, int prec, int nrhs,
                                  int parity, int dagger, const qcuMrhsGeom *g);
 
+/* ---- two-row SU(3) links (12 reals per link) -------------------------------
+ * Gauge compression of the native layout for nColors = 3: each link is kept
+ * as its first two rows, [2][S][8][2][3] in prec, and the kernel rebuilds
+ * row 2 = conj(row 0 x row 1), which holds for SU(3). Link bytes drop by a
+ * third. No counterpart in the reference, which loads full links
+ * (include/kernels/storageLoadStore/loadAndStore.cuh:6-15; its
+ * loadGauge2 at :17-27 is unused).
+ * qcuMrhsPackGauge12 takes the user gauge [4][2][S][3][3] (FP64 or FP32),
+ * checks every link (after dividing the time links that cross the t boundary
+ * by t_sign: -1 when antiperiodic time is folded into the links, else +1) is
+ * SU(3) to 1e-5 and fails otherwise; it synchronises the stream.
+ * qcuMrhsDslash12 is qcuMrhsDslash on that layout, with the same t_sign; the
+ * tile (NULL: the one tuned or chosen for the native layout) follows
+ * qcuMrhsDslash. */
+size_t qcuMrhsGauge12Bytes(const qcuMrhsGeom *g, int prec);
+int qcuMrhsPackGauge12(void *dst, int dst_prec, const void *src_user, int src_prec, const qcuMrhsGeom *g, int t_sign,
+                       void *stream);
+int qcuMrhsDslash12(void *out, const void *in, const void *gauge12, int prec, int nrhs, int parity, int dagger,
+                    int t_sign, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
+
 /* ---- face exchange (one dimension split over processes) ------------------
  * No counterpart in the reference, whose multi-process layer is declared but
  * not implemented (include/qcuDesc/qcu_desc.h:53-97, boundary streams at

@@ -54,7 +54,8 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay", "qcuMrhsDslashEx",
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
-                      "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces"]
+                      "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces",
+                      "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12"]
 
 _lib = None
 
@@ -102,6 +103,9 @@ def _declare(lib):
         "qcuMrhsFaceBytes": (ctypes.c_size_t, [G, I, I, I]),
         "qcuMrhsPackFaces": (I, [PP, PP, P, P, I, I, I, I, G, P]),
         "qcuMrhsDslashFaces": (I, [P, PP, PP, P, I, I, I, I, G, P]),
+        "qcuMrhsGauge12Bytes": (ctypes.c_size_t, [G, I]),
+        "qcuMrhsPackGauge12": (I, [P, I, P, I, G, I, P]),
+        "qcuMrhsDslash12": (I, [P, P, P, I, I, I, I, I, G, T, P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])

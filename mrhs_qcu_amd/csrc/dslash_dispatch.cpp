@@ -127,6 +127,7 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.xa = c.xa;
   a.xb = c.xb;
   a.partial = c.partial;
+  a.t_flip = c.t_flip ? 1 : 0;
   a.rx_r = c.rx_r;
   a.rx_alpha = c.rx_alpha;
   a.sign = c.dagger ? -1.0f : 1.0f;
@@ -192,9 +193,11 @@ int launch_dslash(const DslashCall &c, std::string *err)
       if (!c.rx_alpha) return fail(err, "fused CG update: null alpha"), 1;
       if (uintptr_t(c.rx_r) % al) return fail(err, "packed fields must be 16-byte aligned"), 1;
     }
-    if (c.gl != GL_NATIVE) return fail(err, "packed spinors use the native gauge layout"), 1;
+    if (c.gl != GL_NATIVE && c.gl != GL_NATIVE12)
+      return fail(err, "packed spinors use the native (or two-row SU(3)) gauge layout"), 1;
+    if (c.gl == GL_NATIVE12 && c.geom.N != 3) return fail(err, "two-row links need nColors = 3"), 1;
     const int lds = c.tile.lds ? 1 : 0;
-    KernelKey key{SL_PACKED, GL_NATIVE, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
+    KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
     LaunchFn fn = find_kernel(key);
     if (!fn) return fail(err, "no kernel instantiated for this precision / N / tile"), 1;
     a.k = c.k;
@@ -207,7 +210,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     size_t shm = 0;
     if (lds) {
       const size_t nsite_max = size_t((B - 1) / a.P + 2);
-      shm = nsite_max * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
+      shm = nsite_max * 8 * link_cplx(c.gl, c.geom.N) * size_t(bytes_of(c.prec_g));
       if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
     }
     if (c.tile.occ > 0) {

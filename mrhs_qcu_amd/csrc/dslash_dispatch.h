@@ -92,6 +92,8 @@ struct DslashCall {
   // Fused CG update r -= alpha out (FP32 or FP64 packed output only).
   void *rx_r = nullptr;
   const double *rx_alpha = nullptr;
+  // GL_NATIVE12: time links across the t boundary carry a factor -1.
+  int t_flip = 0;
   hipStream_t stream = nullptr;
 };
 

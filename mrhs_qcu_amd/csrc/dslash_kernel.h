@@ -33,7 +33,12 @@ enum SpinorLayout : int {
 enum GaugeLayout : int {
   GL_NATIVE = 0,   // [2][S][8][N][N]: per site 4 forward links then 4 daggered backward links
   GL_REF_SOA = 1,  // reference SoA [4][2][N*N][S]; backward links daggered on the fly
+  GL_NATIVE12 = 2, // SU(3) only: [2][S][8][2][3], the native links as their first two rows;
+                   // row 2 = conj(row 0 x row 1) is rebuilt in registers
 };
+
+// Stored complex numbers per link of a gauge layout.
+__host__ __device__ constexpr int link_cplx(int GL, int N) { return GL == GL_NATIVE12 ? 2 * N : N * N; }
 
 constexpr int kMaxPtrs = 32;  // RHS per launch for the pointer layouts
 struct PtrPack {
@@ -72,6 +77,8 @@ struct DslashArgs {
   uint32_t nblocks;
   FastDiv div_P, div_subLx, div_row, div_tb, div_zb, div_nzb, div_S, div_Ly;
   int ystream;  // traversal (t_out, z_out, y, t_in, z_in, x/2) instead of (t_out, z_out, t_in, z_in, y, x/2)
+  int t_flip;   // GL_NATIVE12: the time links that cross the t boundary carry a factor -1 (antiperiodic
+                // time folded into the links), which two stored rows cannot express
   int aos_vec;  // pointer layouts: every buffer 16-byte aligned (16-byte loads and stores)
 };
 
@@ -180,6 +187,8 @@ struct SiteGeo {
   // with a cross-lane read instead of a second load; xown marks the lanes
   // that load it themselves (first site of a row or of a wavefront).
   int xown, xsrc;
+  // GL_NATIVE12 with t_flip: negate the rebuilt T+ link (t = Lt - 1) / T- link (t = 0).
+  int tneg[2];
 };
 
 // 32-bit cross-lane read of lane `src` (ds_bpermute); doubles take two.
@@ -255,6 +264,8 @@ __device__ __forceinline__ SiteGeo site_geometry(uint32_t sl, const DslashArgs &
   g.live[5] = !(a.skip[2] && z == 0);
   g.live[6] = !(a.skip[3] && int(t) == a.Lt - 1);
   g.live[7] = !(a.skip[3] && t == 0);
+  g.tneg[0] = a.t_flip && int(t) == a.Lt - 1;
+  g.tneg[1] = a.t_flip && t == 0;
   return g;
 }
 
@@ -310,6 +321,33 @@ __device__ __forceinline__ void load_link_native(cpx<A> (&V)[N][N], const CG *bl
     for (int j = 0; j < N; ++j) V[i][j] = ld_cvt<A>(p[i * N + j]);
 }
 
+// Compressed SU(3) link: rows 0 and 1 stored, row 2 = conj(row 0 x row 1)
+// (unitarity and det = 1); neg flips the sign of the whole link.
+template <int N, class CG, class A>
+__device__ __forceinline__ void load_link_native12(cpx<A> (&V)[N][N], const CG *blk8, int D, bool neg)
+{
+  static_assert(N == 3, "two-row links are SU(3) only");
+  const CG *p = blk8 + D * 2 * N;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = ld_cvt<A>(p[i * N + j]);
+  // (a x b)_j = a_{j+1} b_{j+2} - a_{j+2} b_{j+1}; row 2 is its conjugate.
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const cpx<A> a1 = V[0][(j + 1) % 3], a2 = V[0][(j + 2) % 3], b1 = V[1][(j + 1) % 3], b2 = V[1][(j + 2) % 3];
+    const A re = fmaA(a1.re, b2.re, fmaA(-a1.im, b2.im, fmaA(-a2.re, b1.re, a2.im * b1.im)));
+    const A im = fmaA(a1.re, b2.im, fmaA(a1.im, b2.re, fmaA(-a2.re, b1.im, -(a2.im * b1.re))));
+    V[2][j] = cpx<A>{re, -im};
+  }
+  if (neg) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) V[i][j] = cpx<A>{-V[i][j].re, -V[i][j].im};
+  }
+}
+
 // Reference SoA gauge [4][2][N*N][S]: forward U_mu(x) at x's parity, backward
 // U_mu(x-mu) at the neighbour (opposite parity), conjugate-transposed here.
 template <int N, class CG, class A>
@@ -336,6 +374,8 @@ template <int D, int N, int GL, class CG, class A>
 __device__ __forceinline__ void fetch_link(cpx<A> (&V)[N][N], const DslashArgs &a, const SiteGeo &g, const CG *blk8)
 {
   if constexpr (GL == GL_NATIVE) load_link_native<N, CG, A>(V, blk8, native_link_slot(D));
+  else if constexpr (GL == GL_NATIVE12)
+    load_link_native12<N, CG, A>(V, blk8, native_link_slot(D), D >= 6 ? g.tneg[D - 6] != 0 : false);
   else load_link_refsoa<N, CG, A>(V, a, D, g.idx, g.nbr[D]);
 }
 
@@ -430,6 +470,9 @@ template <> struct Pair<c16> {
     return swap(v) * m;
   }
   __device__ __forceinline__ static float dot(V a, V b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+  __device__ __forceinline__ static f2v to_f(V v) { return __builtin_convertvector(v, f2v); }
+  __device__ __forceinline__ static V from_f(f2v v) { return __builtin_convertvector(v, V); }
+  __device__ __forceinline__ static V neg(V v) { return __builtin_bit_cast(V, __builtin_bit_cast(uint32_t, v) ^ 0x80008000u); }
 };
 
 template <> struct Pair<cb16> {
@@ -465,6 +508,9 @@ template <> struct Pair<cb16> {
   {
     return __builtin_amdgcn_fdot2_f32_bf16(a, b, c, false);
   }
+  __device__ __forceinline__ static f2v to_f(V v) { return __builtin_convertvector(v, f2v); }
+  __device__ __forceinline__ static V from_f(f2v v) { return __builtin_convertvector(v, V); }
+  __device__ __forceinline__ static V neg(V v) { return __builtin_bit_cast(V, __builtin_bit_cast(uint32_t, v) ^ 0x80008000u); }
 };
 
 // acc += hop in direction D for one RHS (16-bit storage).
@@ -525,14 +571,51 @@ __device__ __forceinline__ void load_link_p(typename Pair<C>::V (&V)[N][N], cons
     for (int j = 0; j < N; ++j) V[i][j] = Pair<C>::load(p[i * N + j]);
 }
 
-template <class C, int D, int N, int R, bool LL>
+// Link of direction D as 16-bit pairs: native layout as stored; two-row
+// SU(3) layout with row 2 = conj(row 0 x row 1) formed in FP32 and rounded
+// back to the storage type (and the whole link negated where t_flip asks).
+template <class C, int N, int GL>
+__device__ __forceinline__ void load_link_pg(typename Pair<C>::V (&V)[N][N], const C *blk8, int D, const SiteGeo &g)
+{
+  if constexpr (GL == GL_NATIVE) {
+    load_link_p<C, N>(V, blk8, native_link_slot(D));
+  } else {
+    static_assert(N == 3, "two-row links are SU(3) only");
+    using P = Pair<C>;
+    const C *p = blk8 + native_link_slot(D) * 2 * N;
+    f2v f[2][3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        V[i][j] = P::load(p[i * N + j]);
+        f[i][j] = P::to_f(V[i][j]);
+      }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const f2v a1 = f[0][(j + 1) % 3], a2 = f[0][(j + 2) % 3], b1 = f[1][(j + 1) % 3], b2 = f[1][(j + 2) % 3];
+      const float re = fmaf(a1.x, b2.x, fmaf(-a1.y, b2.y, fmaf(-a2.x, b1.x, a2.y * b1.y)));
+      const float im = fmaf(a1.x, b2.y, fmaf(a1.y, b2.x, fmaf(-a2.x, b1.y, -(a2.y * b1.x))));
+      const f2v r2 = {re, -im};
+      V[2][j] = P::from_f(r2);
+    }
+    if (D >= 6 && g.tneg[D >= 6 ? D - 6 : 0]) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) V[i][j] = P::neg(V[i][j]);
+    }
+  }
+}
+
+template <class C, int D, int N, int R, bool LL, int GL>
 __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
                                         const C *blk8, float s, typename Pair<C>::V (&psi_cur)[R][4][N],
                                         typename Pair<C>::V (&V_cur)[N][N], typename Pair<C>::V (&psi_nxt)[R][4][N],
                                         typename Pair<C>::V (&V_nxt)[N][N])
 {
   if constexpr (D == 0) {
-    if constexpr (!LL) load_link_p<C, N>(V_nxt, blk8, native_link_slot(1));
+    if constexpr (!LL) load_link_pg<C, N, GL>(V_nxt, blk8, 1, g);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -544,10 +627,10 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
     if (g.xown) load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[1], slot);
   } else if constexpr (D + 1 < 8) {
     load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[D + 1], slot);
-    if constexpr (!LL) load_link_p<C, N>(V_nxt, blk8, native_link_slot(D + 1));
+    if constexpr (!LL) load_link_pg<C, N, GL>(V_nxt, blk8, D + 1, g);
   }
   if (g.live[D]) {
-    if constexpr (LL) load_link_p<C, N>(V_cur, blk8, native_link_slot(D));
+    if constexpr (LL) load_link_pg<C, N, GL>(V_cur, blk8, D, g);
     typename Pair<C>::V Vc[N][N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -559,9 +642,9 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
 
   if constexpr (D + 1 < 8) {
     if constexpr (LL)
-      sweep_p<C, D + 1, N, R, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_cur, psi_cur, V_nxt);
+      sweep_p<C, D + 1, N, R, LL, GL>(acc, a, g, slot, blk8, s, psi_nxt, V_cur, psi_cur, V_nxt);
     else
-      sweep_p<C, D + 1, N, R, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+      sweep_p<C, D + 1, N, R, LL, GL>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
   }
 }
 
@@ -619,11 +702,12 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
   }
 
   const CG *blk8 = nullptr;
-  if constexpr (GL == GL_NATIVE && LDS) {
+  constexpr int LC = link_cplx(GL, N);  // stored complex numbers per link
+  if constexpr ((GL == GL_NATIVE || GL == GL_NATIVE12) && LDS) {
     static_assert(!SITE_MINOR, "LDS link staging needs site-major lanes");
     // Stage the 8 links of every site of this workgroup once; all k/R lanes of
     // a site then read them from LDS (broadcast) instead of from L1/L2.
-    constexpr int chunk16 = (8 * N * N * int(sizeof(CG))) / 16;
+    constexpr int chunk16 = (8 * LC * int(sizeof(CG))) / 16;
     const uint32_t first = fdiv(lane0, a.div_P);
     uint32_t lastl = lane0 + blockDim.x - 1;
     if (lastl >= a.nlanes) lastl = a.nlanes - 1;
@@ -637,7 +721,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
     }
     __syncthreads();
-    if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * N * N;
+    if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * LC;
   }
   A nrm[R];
 #pragma unroll
@@ -653,8 +737,8 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       g.xown = (lw < xstride || g.xh == 0) ? 1 : 0;
       g.xsrc = g.xown ? lw : lw - xstride;
     }
-    if constexpr (GL == GL_NATIVE && !LDS)
-      blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * N * N;
+    if constexpr ((GL == GL_NATIVE || GL == GL_NATIVE12) && !LDS)
+      blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * LC;
 
     const A s = A(a.sign);
     cpx<A> acc[R][4][N];
@@ -665,17 +749,17 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};  // x + (-0) == x: folds
 
-    if constexpr (SL == SL_PACKED && GL == GL_NATIVE && std::is_same<CS, CG>::value &&
+    if constexpr (SL == SL_PACKED && (GL == GL_NATIVE || GL == GL_NATIVE12) && std::is_same<CS, CG>::value &&
                   (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) && std::is_same<A, float>::value) {
       using PV = typename Pair<CS>::V;
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor_p<CS, N, R>(psi0, a, g.nbr[0], slot);
-      if constexpr (!LDS) load_link_p<CS, N>(V0, blk8, native_link_slot(0));
-      sweep_p<CS, 0, N, R, LDS>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
+      if constexpr (!LDS) load_link_pg<CS, N, GL>(V0, blk8, 0, g);
+      sweep_p<CS, 0, N, R, LDS, GL>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
-      constexpr bool LL = LDS && GL == GL_NATIVE;
+      constexpr bool LL = LDS && (GL == GL_NATIVE || GL == GL_NATIVE12);
       if constexpr (!LL) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
       sweep<0, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
     }

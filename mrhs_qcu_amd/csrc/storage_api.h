@@ -19,6 +19,11 @@ int unpack_spinors(void *const *dst, int dst_prec, const void *src, int src_prec
 // user gauge [4][2][S][N][N] (FP64 or FP32) -> native [2][S][8][N][N] in dst_prec.
 int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, const int *L, hipStream_t st,
                std::string *err);
+// native FP64 SU(3) gauge [2][S][8][3][3] -> two-row layout [2][S][8][2][3] in
+// dst_prec; links crossing the t boundary divided by t_sign; *bad (device int)
+// += the number of links further than tol from SU(3).
+int compress_gauge12(void *dst, int dst_prec, const void *native64, const int *L, int t_sign, double tol, int *bad,
+                     hipStream_t st, std::string *err);
 // n complex numbers, element-wise precision conversion.
 int convert_complex(void *dst, int dst_prec, const void *src, int src_prec, long long n, hipStream_t st,
                     std::string *err);

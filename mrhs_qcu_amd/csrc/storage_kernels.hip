@@ -312,6 +312,82 @@ int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, co
 }
 
 // ---------------------------------------------------------------------------
+// SU(3) two-row compression: native FP64 [2][S][8][3][3] -> [2][S][8][2][3] in
+// the destination precision. One lane per (parity, site, slot). A link that
+// crosses the t boundary (slot 3 at t = Lt - 1, slot 7 at t = 0) is divided by
+// t_sign first. Every link must then be in SU(3) to within tol: rows 0 and 1
+// orthonormal and row 2 = conj(row 0 x row 1); each violation counts into *bad.
+template <class CD>
+__global__ void __launch_bounds__(256) gauge_compress12_kernel(CD *__restrict__ dst, const c64 *__restrict__ src,
+                                                               int subLx, int Ly, int Lz, int Lt, int t_sign,
+                                                               double tol, long long nlinks, int *bad)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nlinks) return;
+  const long long S = 1LL * subLx * Ly * Lz * Lt;
+  const int slot = int(i % 8);
+  const long long s = (i / 8) % S;
+  const int t = int(s / (1LL * subLx * Ly * Lz));
+  const bool crosses = (slot == 3 && t == Lt - 1) || (slot == 7 && t == 0);
+  const double sg = (crosses && t_sign < 0) ? -1.0 : 1.0;
+  const c64 *u = src + size_t(i) * 9;
+  double re[3][3], im[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      re[r][c] = sg * u[r * 3 + c].x;
+      im[r][c] = sg * u[r * 3 + c].y;
+    }
+  double err = 0.0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = a; b < 2; ++b) {
+      double dr = 0.0, di = 0.0;  // <row a, row b>
+      for (int c = 0; c < 3; ++c) {
+        dr += re[a][c] * re[b][c] + im[a][c] * im[b][c];
+        di += im[a][c] * re[b][c] - re[a][c] * im[b][c];
+      }
+      err = fmax(err, fabs(dr - (a == b ? 1.0 : 0.0)) + fabs(di));
+    }
+  for (int j = 0; j < 3; ++j) {
+    const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    const double cr = re[0][j1] * re[1][j2] - im[0][j1] * im[1][j2] - (re[0][j2] * re[1][j1] - im[0][j2] * im[1][j1]);
+    const double ci = re[0][j1] * im[1][j2] + im[0][j1] * re[1][j2] - (re[0][j2] * im[1][j1] + im[0][j2] * re[1][j1]);
+    err = fmax(err, fabs(cr - re[2][j]) + fabs(-ci - im[2][j]));
+  }
+  if (!(err <= tol)) atomicAdd(bad, 1);
+  CD *d = dst + size_t(i) * 6;
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 3; ++c) d[r * 3 + c] = StCvt<CD>::go(cpx<double>{re[r][c], im[r][c]});
+}
+
+int compress_gauge12(void *dst, int dst_prec, const void *native64, const int *L, int t_sign, double tol, int *bad,
+                     hipStream_t st, std::string *err)
+{
+  const long long S = 1LL * L[0] / 2 * L[1] * L[2] * L[3];
+  const long long nlinks = 2LL * S * 8;
+  const unsigned nblk = unsigned((nlinks + 255) / 256);
+  const c64 *src = static_cast<const c64 *>(native64);
+#define QM_C12(CD)                                                                                               \
+  hipLaunchKernelGGL((gauge_compress12_kernel<CD>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst), src, \
+                     L[0] / 2, L[1], L[2], L[3], t_sign, tol, nlinks, bad)
+  switch (dst_prec) {
+    case PREC_DOUBLE: QM_C12(c64); break;
+    case PREC_SINGLE: QM_C12(c32); break;
+    case PREC_HALF: QM_C12(c16); break;
+    case PREC_BF16: QM_C12(cb16); break;
+    default:
+      if (err) *err = "unsupported gauge precision";
+      return 1;
+  }
+#undef QM_C12
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    if (err) *err = std::string("gauge compression launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Element-wise precision conversion of a contiguous array of complex numbers.
 template <class CD, class CSrc>
 __global__ void __launch_bounds__(256) convert_kernel(CD *__restrict__ dst, const CSrc *__restrict__ src,

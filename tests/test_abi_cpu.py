@@ -74,6 +74,26 @@ def test_argument_validation_without_gpu(lib):
     assert "null batch handle" in lib.qcuMrhsLastError().decode()
 
 
+def test_gauge12_sizes_and_validation(lib):
+    """Two-row links: 6 complex per link, SU(3) only; bad arguments are
+    rejected before anything reaches the GPU."""
+    from mrhs_qcu_amd import _lib
+
+    g3 = _lib.geom((8, 4, 4, 8), 3)
+    S = 4 * 4 * 4 * 8
+    assert lib.qcuMrhsGauge12Bytes(g3, _lib.SINGLE) == S * 2 * 8 * 6 * 8
+    assert lib.qcuMrhsGauge12Bytes(g3, _lib.HALF) * 3 == lib.qcuMrhsGaugeBytes(g3, _lib.HALF) * 2
+    g2 = _lib.geom((8, 4, 4, 8), 2)
+    assert lib.qcuMrhsGauge12Bytes(g2, _lib.SINGLE) == 0
+    assert lib.qcuMrhsPackGauge12(16, _lib.SINGLE, 16, _lib.DOUBLE, g2, -1, None) != 0
+    assert "nColors = 3" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsPackGauge12(16, _lib.SINGLE, 16, _lib.DOUBLE, g3, 0, None) != 0
+    assert "t_sign" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash12(16, 16, 16, _lib.SINGLE, 2, 0, 0, 1, g2, _lib.tile(), None) != 0
+    assert "nColors = 3" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash12(16, 16, 16, _lib.SINGLE, 2, 0, 0, 2, g3, _lib.tile(), None) != 0
+
+
 def test_face_sizes_and_validation(lib):
     """Face buffers: F sites of one parity x 2 spins x N colours x nrhs; a
     split dimension needs both face pointers."""
