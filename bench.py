@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--tile", type=str, default=None,
                     help="fixed tile R,block,lds,tblock[,zblock,hints,occupancy,traversal] for the headline kernel "
                          "(skips the tuner)")
-    ap.add_argument("--faces-only", action="store_true", help="only the split-lattice face exchange timing")
+    ap.add_argument("--faces-only", action="store_true",
+                    help="only the split-lattice face exchange and clover timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
     ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
@@ -268,6 +269,39 @@ def faces_benchmark(lib, _lib, torch, L=(24, 24, 24, 24), grid=(1, 1, 1, 2), N=3
             "faces_GBs": faces_bytes / (ms_faces * 1e-3) / 1e9}
 
 
+def clover_benchmark(lib, _lib, torch, L=(24, 24, 24, 48), N=3, k=16, steps=200, warmup=20):
+    """Clover term A(x) psi(x) on one parity, 24^3x48, 16 RHS, FP32 (field
+    built once from the links, coeff -0.1 arbitrary): time per application
+    and its bytes (read in, write out, read the 8N^2 clover entries of each
+    site once) over the time."""
+    from mrhs_qcu_amd import fields
+
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    geom = _lib.geom(L, N)
+    S = fields.half_volume(L)
+    gu = torch.view_as_real(fields.gauge_field_device(L, N, seed=0, device=dev)).reshape(-1)
+    cl = torch.empty(lib.qcuMrhsCloverBytes(geom, _lib.SINGLE) // 4, dtype=torch.float32, device=dev)
+    _lib.check(lib.qcuMrhsCloverCompute(cl.data_ptr(), None, _lib.SINGLE, gu.data_ptr(), _lib.DOUBLE, -0.1, geom, sp))
+    del gu
+    a = torch.randn(S * 4 * N * k * 2, device=dev, dtype=torch.float32)
+    b = torch.empty_like(a)
+    for i in range(warmup):
+        _lib.check(lib.qcuMrhsCloverApply(b.data_ptr(), a.data_ptr(), cl.data_ptr(), _lib.SINGLE, k, i & 1, geom, sp))
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for i in range(steps):
+        lib.qcuMrhsCloverApply(b.data_ptr(), a.data_ptr(), cl.data_ptr(), _lib.SINGLE, k, i & 1, geom, sp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nbytes = S * (k * 4 * N * 8 * 2 + 8 * N * N * 8)
+    return {"workload": f"su{N}_clover_apply_{'x'.join(map(str, L))}_nrhs{k}_fp32", "ms_per_step": ms,
+            "sites_rhs_per_s": S * k / (ms * 1e-3), "hbm_frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def batch_benchmark(lib, _lib, torch, steps=50, warmup=5):
     """Uneven batched workload in one HIP graph (SU(2) 16^4 nrhs=8, SU(3)
     24^3x48 nrhs=12, SU(4) 16^3x32 nrhs=24, SU(5) 12^3x24 nrhs=6, FP32, tuned
@@ -428,6 +462,7 @@ def main():
         return
     if args.faces_only:
         print(json.dumps(faces_benchmark(lib, _lib, torch)), flush=True)
+        print(json.dumps(clover_benchmark(lib, _lib, torch)), flush=True)
         return
     if args.cg_only:
         solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",)}[args.cg_solver]
@@ -489,6 +524,7 @@ def main():
             "tile": t3}
         extra["batch"] = batch_benchmark(lib, _lib, torch)
         extra["faces"] = faces_benchmark(lib, _lib, torch)
+        extra["clover"] = clover_benchmark(lib, _lib, torch)
         extra["qcu_h_end_to_end"] = qcu_h_end_to_end(lib, _lib, torch)
         line["extra"] = extra
         if not args.no_cg:

@@ -125,6 +125,28 @@ int qcuMrhsPackGauge12(void *dst, int dst_prec, const void *src_user, int src_pr
 int qcuMrhsDslash12(void *out, const void *in, const void *gauge12, int prec, int nrhs, int parity, int dagger,
                     int t_sign, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
 
+/* ---- clover term -----------------------------------------------------------
+ * The operator DSLASH_CLOVER names (include/qcu_enum.h:12 of the reference,
+ * which reserves 72 complex numbers per site for it,
+ * include/basic_data/qcu_point.cuh:98-101, and implements nothing more):
+ *   A(x) = 1 + coeff * sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+ *   sigma_{mu nu} = (i/2)[gamma_mu, gamma_nu] (DeGrand-Rossi, the basis of the
+ *   hopping term), F_{mu nu} = (Q - Q^+)/(8i) with Q the sum of the four
+ *   plaquettes of the (mu, nu) plane around x (no trace subtraction).
+ * A is two Hermitian (2N x 2N) blocks per site (spins {0,1} and {2,3});
+ * field layout [2 parity][S][2][2N][2N] complex in prec, 8 N^2 complex per
+ * site. For the Wilson-clover operator M = A - kappa D, coeff = -kappa c_sw / 2
+ * in the common normalisation.
+ * qcuMrhsCloverCompute builds A (and, with clover_inv non-NULL, A^-1 blockwise)
+ * from the user gauge [4][2][S][N][N] (FP64 or FP32), arithmetic in FP64;
+ * synchronous. qcuMrhsCloverApply: out = C in on the sites of `parity`, C the
+ * given field (A or A^-1), packed spinors in prec. nColors 2..5. */
+size_t qcuMrhsCloverBytes(const qcuMrhsGeom *g, int prec);
+int qcuMrhsCloverCompute(void *clover, void *clover_inv, int prec, const void *gauge_user, int gauge_prec,
+                         double coeff, const qcuMrhsGeom *g, void *stream);
+int qcuMrhsCloverApply(void *out, const void *in, const void *clover, int prec, int nrhs, int parity,
+                       const qcuMrhsGeom *g, void *stream);
+
 /* ---- face exchange (one dimension split over processes) ------------------
  * No counterpart in the reference, whose multi-process layer is declared but
  * not implemented (include/qcuDesc/qcu_desc.h:53-97, boundary streams at

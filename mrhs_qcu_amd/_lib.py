@@ -55,7 +55,8 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
                       "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces",
-                      "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12"]
+                      "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
+                      "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply"]
 
 _lib = None
 
@@ -106,6 +107,9 @@ def _declare(lib):
         "qcuMrhsGauge12Bytes": (ctypes.c_size_t, [G, I]),
         "qcuMrhsPackGauge12": (I, [P, I, P, I, G, I, P]),
         "qcuMrhsDslash12": (I, [P, P, P, I, I, I, I, I, G, T, P]),
+        "qcuMrhsCloverBytes": (ctypes.c_size_t, [G, I]),
+        "qcuMrhsCloverCompute": (I, [P, P, I, P, I, D, G, P]),
+        "qcuMrhsCloverApply": (I, [P, P, P, I, I, I, G, P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])

@@ -117,3 +117,65 @@ def to_prec(arr, prec, dev):
     t = dev_tensor(arr.astype(np.complex128), dev)
     dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
     return t.to(dt)
+
+
+def sigma(mu, nu):
+    """sigma_{mu nu} = (i/2)[gamma_mu, gamma_nu]."""
+    return 0.5j * (GAMMA[mu] @ GAMMA[nu] - GAMMA[nu] @ GAMMA[mu])
+
+
+def gauge_transform(gauge, L, om):
+    """U_mu(x) -> om(x) U_mu(x) om(x+mu)^+ for om [2][S][N][N] (per parity, site)."""
+    xh, y, z, t = fields.site_coords(L)
+    out = np.empty_like(gauge)
+    for p in range(2):
+        X = 2 * xh + ((p + y + z + t) & 1)
+        for mu in range(4):
+            c = [X.copy(), y.copy(), z.copy(), t.copy()]
+            c[mu] = (c[mu] + 1) % L[mu]
+            nidx = cb_index(L, *c)
+            out[mu, p] = om[p] @ gauge[mu, p] @ np.conj(np.swapaxes(om[1 - p, nidx], -1, -2))
+    return out
+
+
+def textbook_clover(gauge, L, coeff):
+    """A(x) = 1 + coeff sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x) from full
+    coordinates, F = (Q - Q^+)/(8i), Q the four plaquettes of the plane
+    around x; returned as the full 4N x 4N matrix per site, [2][S][4N][4N]
+    with index spin * N + colour."""
+    N = gauge.shape[-1]
+    S = fields.half_volume(L)
+    xh, y, z, t = fields.site_coords(L)
+    out = np.zeros((2, S, 4 * N, 4 * N), dtype=np.complex128)
+
+    def dag(m):
+        return np.conj(np.swapaxes(m, -1, -2))
+
+    for p in range(2):
+        X = 2 * xh + ((p + y + z + t) & 1)
+        base = [X, y, z, t]
+
+        def U(mu, shift):
+            c = [(base[d] + shift[d]) % L[d] for d in range(4)]
+            par = (c[0] + c[1] + c[2] + c[3]) & 1
+            idx = cb_index(L, *c)
+            return gauge[mu, par, idx]
+
+        A = np.tile(np.eye(4 * N, dtype=np.complex128), (S, 1, 1))
+        for mu in range(4):
+            for nu in range(mu + 1, 4):
+                e = lambda m, n: [m * (d == mu) + n * (d == nu) for d in range(4)]
+                Q = U(mu, e(0, 0)) @ U(nu, e(1, 0)) @ dag(U(mu, e(0, 1))) @ dag(U(nu, e(0, 0)))
+                Q = Q + U(nu, e(0, 0)) @ dag(U(mu, e(-1, 1))) @ dag(U(nu, e(-1, 0))) @ U(mu, e(-1, 0))
+                Q = Q + dag(U(mu, e(-1, 0))) @ dag(U(nu, e(-1, -1))) @ U(mu, e(-1, -1)) @ U(nu, e(0, -1))
+                Q = Q + dag(U(nu, e(0, -1))) @ U(mu, e(0, -1)) @ U(nu, e(1, -1)) @ dag(U(mu, e(0, 0)))
+                F = (Q - dag(Q)) / 8j
+                A = A + coeff * np.einsum("ab,sij->saibj", sigma(mu, nu), F).reshape(S, 4 * N, 4 * N)
+        out[p] = A
+    return out
+
+
+def clover_blocks(A, N):
+    """[2][S][4N][4N] -> the two diagonal spin-pair blocks [2][S][2][2N][2N]."""
+    B = 2 * N
+    return np.stack([A[..., :B, :B], A[..., B:, B:]], axis=2)

@@ -94,6 +94,19 @@ def test_gauge12_sizes_and_validation(lib):
     assert lib.qcuMrhsDslash12(16, 16, 16, _lib.SINGLE, 2, 0, 0, 2, g3, _lib.tile(), None) != 0
 
 
+def test_clover_sizes_and_validation(lib):
+    from mrhs_qcu_amd import _lib
+
+    g3 = _lib.geom((8, 4, 4, 8), 3)
+    S = 4 * 4 * 4 * 8
+    assert lib.qcuMrhsCloverBytes(g3, _lib.SINGLE) == 2 * S * 72 * 8
+    assert lib.qcuMrhsCloverBytes(_lib.geom((8, 4, 4, 8), 6), _lib.SINGLE) == 0
+    assert lib.qcuMrhsCloverCompute(16, None, _lib.SINGLE, 16, _lib.HALF, 0.1, g3, None) != 0
+    assert "FP64 or FP32" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsCloverApply(16, 32, 16, _lib.SINGLE, 4, 2, g3, None) != 0
+    assert "parity" in lib.qcuMrhsLastError().decode()
+
+
 def test_face_sizes_and_validation(lib):
     """Face buffers: F sites of one parity x 2 spins x N colours x nrhs; a
     split dimension needs both face pointers."""
