@@ -455,12 +455,14 @@ int qcuMrhsDslashEx(void *out, int out_prec, const void *in, const void *x, int 
   double *partial = nullptr;
   const long long nb = dslash_num_blocks(c);
   if (norm2_out) {
-    if (hipMalloc(&partial, size_t(nb) * nrhs * sizeof(double)) != hipSuccess) return api_fail("allocation failed");
+    // [nb][nrhs] partials followed by the reduction's stage buffer
+    if (hipMalloc(&partial, (size_t(nb) + kReduceStages) * nrhs * sizeof(double)) != hipSuccess)
+      return api_fail("allocation failed");
     c.partial = partial;
   }
   int rc = launch_dslash(c, &err);
   if (!rc && norm2_out) {
-    hipError_t e = blas_reduce(partial, nb, nrhs, norm2_out, st);
+    hipError_t e = blas_reduce(partial, nb, nrhs, norm2_out, partial + size_t(nb) * nrhs, st);
     if (e != hipSuccess) rc = 1, err = hipGetErrorString(e);
     if (hipStreamSynchronize(st) != hipSuccess) rc = 1, err = "synchronisation failed";
   }

@@ -25,15 +25,19 @@ hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *
                          const double *gamma, void *x, const double *alpha, const float *sc_p, double *partial,
                          long long n, int k, hipStream_t st);
 hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
-// r = b - Ax (b, Ax FP64; r stored in prec_k), partials |r|^2 and Re <r, p>.
-hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, const void *p, double *partial_rr,
-                         double *partial_rp, long long n, int k, hipStream_t st);
+// r = b - Ax (b FP64, Ax FP64 or, with FP32 r, FP32; difference in FP64, r
+// stored in prec_k; Ax may be r itself), partials |r|^2 and Re <r, p>.
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p,
+                         double *partial_rr, double *partial_rp, long long n, int k, hipStream_t st);
 // xh (FP64) += x + alpha p, then x = 0 (x = NULL: xh += alpha p; p = NULL: no
 // alpha p term).
 hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, const double *alpha, long long n, int k,
                            hipStream_t st);
 hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
-hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st);
+// out[r] = sum over nb workgroups of partial[b][r]; stage: device scratch of
+// kReduceStages * k doubles.
+constexpr int kReduceStages = 256;
+hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, double *stage, hipStream_t st);
 hipError_t cg_alpha(const CgState &s, int k, hipStream_t st);
 hipError_t cg_beta(const CgState &s, int k, double stop2, hipStream_t st);
 hipError_t cg_orth(const CgState &s, int k, hipStream_t st);

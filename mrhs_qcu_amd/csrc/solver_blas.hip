@@ -3,7 +3,7 @@
 // Every kernel is grid-stride with a thread count that is a multiple of k, so
 // each lane keeps one right-hand side for its whole walk and accumulates its
 // per-RHS sums in a register; a workgroup folds them into LDS and writes one
-// partial per RHS: [nblocks][k], summed by cg_reduce_kernel. No global atomics,
+// partial per RHS: [nblocks][k], summed by blas_reduce (two passes). No global atomics,
 // so every reduction is deterministic.
 #include <hip/hip_runtime.h>
 
@@ -84,12 +84,12 @@ __global__ void __launch_bounds__(kBlasBlock) norm2_kernel(const C *__restrict__
   block_partial(acc, rhs, k, partial);
 }
 
-// r = b - Ax (FP64) ; partial |r|^2
-// r = b - Ax computed in FP64 and stored in the Krylov precision CK;
-// partials |r|^2 (of the FP64 values) and Re <r, p> (of the stored r).
-template <class CK>
-__global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *__restrict__ r, const c64 *__restrict__ b,
-                                                              const c64 *__restrict__ Ax, const CK *__restrict__ p,
+// r = b - Ax computed in FP64 (Ax stored as CA) and stored in the Krylov
+// precision CK; partials |r|^2 (of the FP64 values) and Re <r, p> (of the
+// stored r). Ax may be r itself (each element is read before it is written).
+template <class CK, class CA>
+__global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *r, const c64 *__restrict__ b,
+                                                              const CA *Ax, const CK *__restrict__ p,
                                                               double *partial_rr, double *partial_rp, long long n,
                                                               int k)
 {
@@ -99,8 +99,9 @@ __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *__restrict__ r
   const int rhs = int(g % k);
   double acc = 0.0, dot = 0.0;
   for (long long e = g; e < n; e += T) {
-    const c64 bb = b[e], aa = Ax[e];
-    const cpx<double> rr{bb.x - aa.x, bb.y - aa.y};
+    const c64 bb = b[e];
+    const cpx<double> aa = ld_cvt<double>(Ax[e]);
+    const cpx<double> rr{bb.x - aa.re, bb.y - aa.im};
     const CK rs = StCvt<CK>::go(rr);
     r[e] = rs;
     acc += rr.re * rr.re + rr.im * rr.im;
@@ -155,21 +156,35 @@ __global__ void __launch_bounds__(kBlasBlock) zero_kernel(C *__restrict__ v, lon
     v[e] = StCvt<C>::go(cpx<float>{0.f, 0.f});
 }
 
-// out[r] = sum_b partial[b][r] for r < k, one workgroup per RHS.
-__global__ void __launch_bounds__(256) reduce_kernel(const double *__restrict__ partial, long long nb, int k,
-                                                     double *__restrict__ out)
+// out[r] = sum_b partial[b][r] in two passes. Pass 1: G workgroups walk the
+// flat [nb][k] array with coalesced loads; the thread count per workgroup is
+// a multiple of k, so every thread keeps one RHS; each workgroup folds its
+// threads per RHS in a fixed order into stage[G][k]. Pass 2: one thread per
+// RHS sums the G stage values in order. Deterministic, no atomics.
+__global__ void __launch_bounds__(256) reduce_pass1(const double *__restrict__ partial, long long n, int k,
+                                                    double *__restrict__ stage)
 {
   __shared__ double red[256];
-  const int r = blockIdx.x;
+  const long long T = (long long)gridDim.x * blockDim.x;
   double acc = 0.0;
-  for (long long b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[b * k + r];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += T) acc += partial[e];
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+  if ((int)threadIdx.x < k) {
+    double sum = 0.0;
+    for (int t = threadIdx.x; t < (int)blockDim.x; t += k) sum += red[t];
+    stage[size_t(blockIdx.x) * k + threadIdx.x] = sum;
   }
-  if (threadIdx.x == 0) out[r] = red[0];
+}
+
+__global__ void __launch_bounds__(256) reduce_pass2(const double *__restrict__ stage, int G, int k,
+                                                    double *__restrict__ out)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  double sum = 0.0;
+  for (int b = 0; b < G; ++b) sum += stage[size_t(b) * k + r];
+  out[r] = sum;
 }
 
 // alpha = rr / pAp for live RHS; pnorm = sqrt(sum |p|^2).
@@ -320,14 +335,17 @@ hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int
   return hipGetLastError();
 }
 
-hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, const void *p, double *partial_rr,
-                         double *partial_rp, long long n, int k, hipStream_t st)
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p,
+                         double *partial_rr, double *partial_rp, long long n, int k, hipStream_t st)
 {
   if (prec_k == PREC_DOUBLE)
-    hipLaunchKernelGGL(residual_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
+    hipLaunchKernelGGL((residual_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
                        static_cast<const c64 *>(Ax), static_cast<const c64 *>(p), partial_rr, partial_rp, n, k);
+  else if (prec_ax == PREC_SINGLE)
+    hipLaunchKernelGGL((residual_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
+                       static_cast<const c32 *>(Ax), static_cast<const c32 *>(p), partial_rr, partial_rp, n, k);
   else
-    hipLaunchKernelGGL(residual_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
+    hipLaunchKernelGGL((residual_kernel<c32, c64>), QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
                        static_cast<const c64 *>(Ax), static_cast<const c32 *>(p), partial_rr, partial_rp, n, k);
   return hipGetLastError();
 }
@@ -355,9 +373,16 @@ hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, hipStream_t st)
+hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, double *stage, hipStream_t st)
 {
-  hipLaunchKernelGGL(reduce_kernel, dim3(k), dim3(256), 0, st, partial, nb, k, out);
+  if (k < 1 || k > kBlasMaxRhs) return hipErrorInvalidValue;
+  const int block = (256 / k) * k;
+  const long long n = nb * k;
+  long long G = (n + block * 16LL - 1) / (block * 16LL);
+  if (G > kReduceStages) G = kReduceStages;
+  if (G < 1) G = 1;
+  hipLaunchKernelGGL(reduce_pass1, dim3(unsigned(G)), dim3(block), 0, st, partial, n, k, stage);
+  hipLaunchKernelGGL(reduce_pass2, dim3(unsigned((k + 255) / 256)), dim3(256), 0, st, stage, int(G), k, out);
   return hipGetLastError();
 }
 

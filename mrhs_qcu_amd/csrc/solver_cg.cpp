@@ -163,13 +163,15 @@ struct Solver {
     return dslash_num_blocks(c);
   }
 
-  // out = M^ in (dagger 0) or M^+ in (dagger 1), FP64, using tmp (odd parity).
+  // out = M^ in (dagger 0) or M^+ in (dagger 1) in prec, using tmp (odd parity).
+  bool apply_m(void *out, const void *in, void *tmp, const void *gauge, int dagger, int prec)
+  {
+    return dslash(tmp, in, nullptr, 0.0, 1.0, gauge, prec, prec, 1, dagger, nullptr, nullptr, nullptr, false) &&
+           dslash(out, tmp, in, 1.0, -kappa * kappa, gauge, prec, prec, 0, dagger, nullptr, nullptr, nullptr, false);
+  }
   bool apply_m64(void *out, const void *in, void *tmp, const void *gauge, int dagger)
   {
-    return dslash(tmp, in, nullptr, 0.0, 1.0, gauge, PREC_DOUBLE, PREC_DOUBLE, 1, dagger, nullptr, nullptr, nullptr,
-                  false) &&
-           dslash(out, tmp, in, 1.0, -kappa * kappa, gauge, PREC_DOUBLE, PREC_DOUBLE, 0, dagger, nullptr, nullptr,
-                  nullptr, false);
+    return apply_m(out, in, tmp, gauge, dagger, PREC_DOUBLE);
   }
 };
 
@@ -252,6 +254,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   s.partial_cap = size_t(nb_max) * k;
   s.partial = mem.alloc<double>(s.partial_cap, err, &ok);
   double *partial_p = mem.alloc<double>(size_t(nb_blas) * k, err, &ok);
+  double *stage = mem.alloc<double>(size_t(kReduceStages) * k, err, &ok);
   CgState cs;
   cs.rr = mem.alloc<double>(k, err, &ok);
   cs.rr_new = mem.alloc<double>(k, err, &ok);
@@ -279,6 +282,26 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // overwrites the scratch fields it times on; they are initialised below).
   if (!tuned(g, ps, k) && !tune_now(t_s, p_s, gs, ps, k, g, st, err)) return 1;
   if (mixed && !tuned(g, PREC_DOUBLE, k) && !tune_now(t64, ax64, gauge64, PREC_DOUBLE, k, g, st, err)) return 1;
+  // Reliable updates taken while every live residual is still far above FP32
+  // resolution (relative residual > 1e-5; the FP32 operator's error is about
+  // 1e-7 |b^|) apply M^+M^ in FP32; the later ones, and every convergence
+  // decision near tol, use FP64. The FP32 vectors live in the FP64 scratch
+  // t64 (x32, t32) and ax64 (rh32); M^+M^ x lands in r.
+  const bool fp32_updates = mixed && prm->tol < 1e-6;
+  void *x32 = t64, *t32 = static_cast<char *>(t64) + bk, *rh32 = ax64;
+  void *g32 = nullptr;
+  if (fp32_updates) {
+    if (ps == PREC_SINGLE) {
+      g32 = gs;
+    } else {
+      g32 = mem.bytes(size_t(s.S) * 2 * 8 * g.N * g.N * prec_bytes(PREC_SINGLE), err, &ok);
+      if (!ok) return 1;
+      std::string e2;
+      if (convert_complex(g32, PREC_SINGLE, gauge64, PREC_DOUBLE, s.S * 2 * 8 * g.N * g.N, st, &e2))
+        return *err = e2, 1;
+    }
+    if (!tuned(g, PREC_SINGLE, k) && !tune_now(t32, x32, g32, PREC_SINGLE, k, g, st, err)) return 1;
+  }
   const bool scaled = mixed && ps == PREC_HALF;
   const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
   const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
@@ -292,7 +315,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // b^ = M^+ b ; |b^|^2
   QM_TRY(s.apply_m64(bhat, b, t64, gauge64, 1));
   QM_HIP(blas_norm2(PREC_DOUBLE, bhat, s.partial, n, k, st));
-  QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.b2, st));
+  QM_HIP(blas_reduce(s.partial, nb_blas, k, cs.b2, stage, st));
   QM_HIP(cg_init_state(cs, k, st));  // rr = b2, beta = 0, done = 0, scales from |b^|
   // x = 0, r = b^, p = r
   QM_HIP(blas_zero(PREC_DOUBLE, x_out, n, k, st));
@@ -339,18 +362,28 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // True residual: x_h += x_k (mixed) + alpha p (the step of the iteration
   // just finished, when `step`), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces the
   // iterated norm, done flags; the Krylov residual restarts from r_h.
-  auto reliable_update = [&](bool step) -> bool {
+  auto reliable_update = [&](bool step, bool use32) -> bool {
     if ((mixed || step) && !hip_ok(blas_accumulate(pk, x_out, mixed ? xk : nullptr, step ? p : nullptr, cs.alpha,
                                                    n, k, st),
                                    err, "accumulate"))
       return false;
-    if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
-    if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
     // r = b^ - M^+M^ x_h straight into the Krylov vector, with |r|^2 and
     // <r, p> (for re-orthogonalising p) from the same pass.
-    if (!hip_ok(blas_residual(pk, r, bhat, ax64, p, s.partial, partial_p, n, k, st), err, "residual")) return false;
-    if (!hip_ok(blas_reduce(s.partial, nb_blas, k, rr_true, st), err, "reduce")) return false;
-    if (!hip_ok(blas_reduce(partial_p, nb_blas, k, cs.rdotp, st), err, "reduce")) return false;
+    if (use32) {
+      std::string e2;
+      if (convert_complex(x32, PREC_SINGLE, x_out, PREC_DOUBLE, n, st, &e2)) return *err = e2, false;
+      if (!s.apply_m(rh32, x32, t32, g32, 0, PREC_SINGLE)) return false;
+      if (!s.apply_m(r, rh32, t32, g32, 1, PREC_SINGLE)) return false;
+      if (!hip_ok(blas_residual(pk, r, bhat, r, PREC_SINGLE, p, s.partial, partial_p, n, k, st), err, "residual"))
+        return false;
+    } else {
+      if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
+      if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
+      if (!hip_ok(blas_residual(pk, r, bhat, ax64, PREC_DOUBLE, p, s.partial, partial_p, n, k, st), err, "residual"))
+        return false;
+    }
+    if (!hip_ok(blas_reduce(s.partial, nb_blas, k, rr_true, stage, st), err, "reduce")) return false;
+    if (!hip_ok(blas_reduce(partial_p, nb_blas, k, cs.rdotp, stage, st), err, "reduce")) return false;
     if (!hip_ok(cg_check_true(cs, rr_true, k, stop2, st), err, "check")) return false;
     ++updates;
     since_update = 0;
@@ -366,7 +399,9 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       rr_ru[i] = rrt[i];
       mx = std::max(mx, b2h[i] > 0 ? std::sqrt(rrt[i] / b2h[i]) : 0.0);
     }
-    if (verbose) fprintf(stderr, "cg iter %d: reliable update %d, true relative residual max %.3e\n", iter, updates, mx);
+    if (verbose)
+      fprintf(stderr, "cg iter %d: reliable update %d (%s operator), true relative residual max %.3e\n", iter, updates,
+              use32 ? "FP32" : "FP64", mx);
     return true;
   };
 
@@ -387,30 +422,31 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
     QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
     QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
-    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.pap, st));
-    QM_HIP(blas_reduce(partial_p, nb_blas, k, cs.pnorm2, st));
+    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.pap, stage, st));
+    QM_HIP(blas_reduce(partial_p, nb_blas, k, cs.pnorm2, stage, st));
     QM_HIP(cg_alpha(cs, k, st));
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
     // Ap = q - kappa^2 D^+_eo t2 stays in registers: the same launch does
     // r -= alpha Ap and the per-RHS |r|^2 partials (out unused).
     QM_TRY(s.dslash(r, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
-    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.rr_new, st));
+    QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.rr_new, stage, st));
     QM_HIP(hipMemcpyAsync(h_rr, cs.rr_new, k * sizeof(double), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));
     ++iter;
     ++since_update;
     bool all_conv = true, drop = false;
-    double mx = 0;
+    double mx = 0, mn2 = 1e300;
     for (int i = 0; i < k; ++i) {
       if (done[i]) continue;
       all_conv = all_conv && h_rr[i] <= stop2 * b2h[i];
       drop = drop || (h_rr[i] < delta2 * rr_ru[i]);
       mx = std::max(mx, b2h[i] > 0 ? std::sqrt(h_rr[i] / b2h[i]) : 0.0);
+      if (b2h[i] > 0) mn2 = std::min(mn2, h_rr[i] / b2h[i]);
     }
     if (verbose && iter % 10 == 0) fprintf(stderr, "cg iter %d: iterated relative residual max %.3e\n", iter, mx);
     if (all_conv || drop || (mixed && since_update >= interval) || iter == max_iter) {
-      QM_TRY(reliable_update(true));
+      QM_TRY(reliable_update(true, fp32_updates && !all_conv && mn2 > 1e-10));
       if (converged || iter == max_iter) break;
       // beta = |r_true|^2 / |r_old|^2 ; p = gamma r + beta p with the old p
       // made orthogonal to r_true.
@@ -427,7 +463,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   }
   // Every iteration ends in a reliable update or in a p update that applied
   // its x step, so x is complete here.
-  if (!converged && since_update > 0) QM_TRY(reliable_update(false));
+  if (!converged && since_update > 0) QM_TRY(reliable_update(false, false));
   QM_HIP(hipMemcpyAsync(rrt.data(), rr_true, k * sizeof(double), hipMemcpyDeviceToHost, st));
   QM_HIP(hipMemcpyAsync(b2h.data(), cs.b2, k * sizeof(double), hipMemcpyDeviceToHost, st));
   QM_HIP(hipStreamSynchronize(st));
