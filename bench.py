@@ -145,12 +145,14 @@ def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0):
 
 
 def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0,
-                 solvers=("fp64", "mixed_fp16")):
+                 solvers=("fp64", "mixed_fp16"), repeats=3):
     """Even-odd preconditioned MRHS CG on M^+M^ to a true relative residual
     tol: pure FP64 vs mixed precision (FP16 dslash storage, FP32 Krylov
     vectors, FP64 reductions and reliable updates), same GPU, same inputs.
     Sources: RHS 0..min(k,4N)-1 unit deltas (spin-colour component r) at
-    seeded even sites, the rest Gaussian noise."""
+    seeded even sites, the rest Gaussian noise. Each solver: one untimed
+    2-iteration solve (tuning, work vectors), then `repeats` timed full solves
+    from x = 0; seconds = the fastest, seconds_all = every one."""
     import numpy as np
 
     from mrhs_qcu_amd import fields
@@ -187,15 +189,23 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
         # work vectors (both kept for the timed solve).
         prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 2, 50, sloppy
         _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp))
-        prm = _lib.CgParam()
-        prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 20000, 50, sloppy
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp))
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        out[name] = {"seconds": wall, "iterations": prm.iterations, "reliable_updates": prm.reliable_updates,
-                     "max_true_rel_residual": max(res), "converged": bool(max(res) <= tol)}
+        # Timed: the full solve from x = 0, `repeats` times; the fastest is
+        # reported (every repeat does the same work and must converge).
+        walls = []
+        for _ in range(repeats):
+            prm = _lib.CgParam()
+            prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 20000, 50, sloppy
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res,
+                                          sp))
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            if not max(res) <= tol:
+                break
+        out[name] = {"seconds": min(walls), "seconds_all": walls, "iterations": prm.iterations,
+                     "reliable_updates": prm.reliable_updates, "max_true_rel_residual": max(res),
+                     "converged": bool(max(res) <= tol)}
         sols[name] = x
     if len(sols) < 2:
         return out
