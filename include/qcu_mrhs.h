@@ -207,6 +207,14 @@ int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, cons
 /* The solver keeps its work vectors between calls (no allocation when the
  * shape repeats) and tunes its operator tiles on first use; this releases
  * the work vectors. */
+/* The same solver for the Wilson-clover operator: on the even checkerboard
+ * M^ = A_ee - kappa^2 D_eo A_oo^-1 D_oe (A, A^-1 from qcuMrhsCloverCompute in
+ * FP64; the hopping normalisation of M = A - kappa D). Solves M^+M^ x = M^+ b
+ * with the same parameters; with a 16-bit sloppy precision A and A^-1 are
+ * also kept in it. nColors 2..5. */
+int qcuMrhsCgSolveClover(void *x, const void *b, const void *gauge_f64, const void *clover_f64,
+                         const void *clover_inv_f64, int nrhs, const qcuMrhsGeom *g, qcuMrhsCgParam *p,
+                         double *true_res, void *stream);
 int qcuMrhsSolverRelease(void);
 
 /* ---- batching ------------------------------------------------------------ */

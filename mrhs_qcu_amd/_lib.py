@@ -56,7 +56,8 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
                       "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces",
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
-                      "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply"]
+                      "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
+                      "qcuMrhsCgSolveClover"]
 
 _lib = None
 
@@ -110,6 +111,7 @@ def _declare(lib):
         "qcuMrhsCloverBytes": (ctypes.c_size_t, [G, I]),
         "qcuMrhsCloverCompute": (I, [P, P, I, P, I, D, G, P]),
         "qcuMrhsCloverApply": (I, [P, P, P, I, I, I, G, P]),
+        "qcuMrhsCgSolveClover": (I, [P, P, P, P, P, I, G, ctypes.POINTER(CgParam), ctypes.POINTER(D), P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "api_internal.h"
+#include "clover.h"
 #include "dslash_dispatch.h"
 #include "qcu.h"
 #include "qcu_mrhs.h"
@@ -163,15 +164,37 @@ struct Solver {
     return dslash_num_blocks(c);
   }
 
-  // out = M^ in (dagger 0) or M^+ in (dagger 1) in prec, using tmp (odd parity).
-  bool apply_m(void *out, const void *in, void *tmp, const void *gauge, int dagger, int prec)
+  // Clover term (both null: Wilson): FP64 fields A and A^-1, and two FP64
+  // scratch fields for the clover form of M^.
+  const void *cl64 = nullptr, *cli64 = nullptr;
+  void *cl_u = nullptr, *cl_a = nullptr;
+
+  bool clover(void *out, const void *in, const void *field, int prec, int parity)
   {
-    return dslash(tmp, in, nullptr, 0.0, 1.0, gauge, prec, prec, 1, dagger, nullptr, nullptr, nullptr, false) &&
-           dslash(out, tmp, in, 1.0, -kappa * kappa, gauge, prec, prec, 0, dagger, nullptr, nullptr, nullptr, false);
+    return hip_ok_s(clover_apply(prec, out, in, field, parity, int(S), k, g.N, st), "clover apply");
+  }
+  bool hip_ok_s(hipError_t e, const char *what)
+  {
+    if (e == hipSuccess) return true;
+    *err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+  }
+
+  // out = M^ in (dagger 0) or M^+ in (dagger 1) in prec, using tmp (odd parity):
+  // Wilson M^ = 1 - kappa^2 D_eo D_oe; with the clover term
+  // M^ = A_ee - kappa^2 D_eo A_oo^-1 D_oe (gauge and clover fields in prec).
+  bool apply_m(void *out, const void *in, void *tmp, const void *gauge, int dagger, int prec,
+               const void *cla = nullptr, const void *clai = nullptr, void *u = nullptr, void *a = nullptr)
+  {
+    if (!dslash(tmp, in, nullptr, 0.0, 1.0, gauge, prec, prec, 1, dagger, nullptr, nullptr, nullptr, false)) return false;
+    if (!cla)
+      return dslash(out, tmp, in, 1.0, -kappa * kappa, gauge, prec, prec, 0, dagger, nullptr, nullptr, nullptr, false);
+    return clover(u, tmp, clai, prec, 1) && clover(a, in, cla, prec, 0) &&
+           dslash(out, u, a, 1.0, -kappa * kappa, gauge, prec, prec, 0, dagger, nullptr, nullptr, nullptr, false);
   }
   bool apply_m64(void *out, const void *in, void *tmp, const void *gauge, int dagger)
   {
-    return apply_m(out, in, tmp, gauge, dagger, PREC_DOUBLE);
+    return apply_m(out, in, tmp, gauge, dagger, PREC_DOUBLE, cl64, cli64, cl_u, cl_a);
   }
 };
 
@@ -209,7 +232,8 @@ static bool tune_now(void *out, void *in, const void *gauge, int prec, int k, co
 }
 
 int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom &g, qcuMrhsCgParam *prm,
-             double *true_res, hipStream_t st, std::string *err)
+             double *true_res, hipStream_t st, std::string *err, const void *clover64 = nullptr,
+             const void *clover_inv64 = nullptr)
 {
   const auto t0 = std::chrono::steady_clock::now();
   if (k < 1 || k > kBlasMaxRhs) return *err = "nrhs must be in 1..256", 1;
@@ -217,6 +241,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   if (ps != PREC_DOUBLE && ps != PREC_SINGLE && ps != PREC_HALF && ps != PREC_BF16)
     return *err = "unsupported sloppy precision", 1;
   if (!(prm->tol > 0.0)) return *err = "tolerance must be positive", 1;
+  const bool clov = clover64 != nullptr;
+  if (clov && (!clover_inv64 || g.N > 5)) return *err = "the clover solve needs A and A^-1 and nColors <= 5", 1;
   const bool mixed = ps != PREC_DOUBLE;
   const int pk = mixed ? PREC_SINGLE : PREC_DOUBLE;  // Krylov precision
   Solver s;
@@ -273,6 +299,33 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   cs.sc_q = mem.alloc<float>(k, err, &ok);
   cs.sc_t2 = mem.alloc<float>(k, err, &ok);
   double *rr_true = mem.alloc<double>(k, err, &ok);
+  // Clover: FP64 scratch for M^ in FP64 (also the loop's fields when the
+  // solve is all FP64), sloppy copies of A and A^-1 and two sloppy fields.
+  const size_t clover_elems = size_t(s.S) * 2 * 8 * g.N * g.N;
+  void *u_s = nullptr, *a_s = nullptr;
+  const void *cl_s = nullptr, *cli_s = nullptr;
+  if (clov) {
+    s.cl64 = clover64;
+    s.cli64 = clover_inv64;
+    s.cl_u = mem.bytes(b64, err, &ok);
+    s.cl_a = mem.bytes(b64, err, &ok);
+    u_s = mixed ? mem.bytes(bs, err, &ok) : s.cl_u;
+    a_s = mixed ? mem.bytes(bs, err, &ok) : s.cl_a;
+    if (mixed) {
+      void *c1 = mem.bytes(clover_elems * prec_bytes(ps), err, &ok);
+      void *c2 = mem.bytes(clover_elems * prec_bytes(ps), err, &ok);
+      if (!ok) return 1;
+      std::string e2;
+      if (convert_complex(c1, ps, clover64, PREC_DOUBLE, clover_elems, st, &e2) ||
+          convert_complex(c2, ps, clover_inv64, PREC_DOUBLE, clover_elems, st, &e2))
+        return *err = e2, 1;
+      cl_s = c1;
+      cli_s = c2;
+    } else {
+      cl_s = clover64;
+      cli_s = clover_inv64;
+    }
+  }
   if (!ok) return 1;
   if (mixed) {
     std::string e2;
@@ -287,7 +340,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // 1e-7 |b^|) apply M^+M^ in FP32; the later ones, and every convergence
   // decision near tol, use FP64. The FP32 vectors live in the FP64 scratch
   // t64 (x32, t32) and ax64 (rh32); M^+M^ x lands in r.
-  const bool fp32_updates = mixed && prm->tol < 1e-6;
+  const bool fp32_updates = mixed && !clov && prm->tol < 1e-6;
   void *x32 = t64, *t32 = static_cast<char *>(t64) + bk, *rh32 = ax64;
   void *g32 = nullptr;
   if (fp32_updates) {
@@ -419,17 +472,30 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   fuse.rx_r = r;
   fuse.rx_alpha = cs.alpha;
   while (!converged && iter < max_iter) {
-    // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>)
+    // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>). Clover:
+    // q = A_ee p - kappa^2 D_eo A_oo^-1 t (the clover applications keep the
+    // per-RHS storage scale of their input, the operator being linear).
     QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
-    QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
+    if (clov) {
+      QM_TRY(s.clover(u_s, t_s, cli_s, ps, 1) && s.clover(a_s, p_s, cl_s, ps, 0));
+      QM_TRY(s.dslash(q_s, u_s, a_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
+    } else {
+      QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
+    }
     QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.pap, stage, st));
     QM_HIP(blas_reduce(partial_p, nb_blas, k, cs.pnorm2, stage, st));
     QM_HIP(cg_alpha(cs, k, st));
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
     // Ap = q - kappa^2 D^+_eo t2 stays in registers: the same launch does
-    // r -= alpha Ap and the per-RHS |r|^2 partials (out unused).
-    QM_TRY(s.dslash(r, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
+    // r -= alpha Ap and the per-RHS |r|^2 partials (out unused). Clover:
+    // Ap = A_ee q - kappa^2 D^+_eo A_oo^-1 t2.
+    if (clov) {
+      QM_TRY(s.clover(u_s, t2_s, cli_s, ps, 1) && s.clover(a_s, q_s, cl_s, ps, 0));
+      QM_TRY(s.dslash(r, u_s, a_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
+    } else {
+      QM_TRY(s.dslash(r, t2_s, q_s, 1.0, kk2, gs, ps, pk, 0, 1, sc_t2, sc_q, nullptr, true, &fuse));
+    }
     QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.rr_new, stage, st));
     QM_HIP(hipMemcpyAsync(h_rr, cs.rr_new, k * sizeof(double), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));
@@ -548,6 +614,20 @@ int qcuMrhsCgSolve(void *x, const void *b, const void *gauge_f64, int nrhs, cons
   if (!geom_from_c(gc, &g, &err)) return api_fail(err);
   if (!x || !b || !gauge_f64 || !p) return api_fail("null argument");
   if (cg_solve(x, b, gauge_f64, nrhs, g, p, true_res, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  return 0;
+}
+
+int qcuMrhsCgSolveClover(void *x, const void *b, const void *gauge_f64, const void *clover_f64,
+                         const void *clover_inv_f64, int nrhs, const qcuMrhsGeom *gc, qcuMrhsCgParam *p,
+                         double *true_res, void *stream)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!x || !b || !gauge_f64 || !clover_f64 || !clover_inv_f64 || !p) return api_fail("null argument");
+  if (cg_solve(x, b, gauge_f64, nrhs, g, p, true_res, static_cast<hipStream_t>(stream), &err, clover_f64,
+               clover_inv_f64))
+    return api_fail(err);
   return 0;
 }
 
