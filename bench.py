@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG comparison")
     ap.add_argument("--cg-only", action="store_true", help="only the CG comparison (printed as one JSON line)")
     ap.add_argument("--cg-L", type=int, nargs=4, default=[32, 32, 32, 64])
+    ap.add_argument("--cg-clover", type=float, default=None, help="with --cg-only: Wilson-clover with this c_sw")
     ap.add_argument("--cg-solver", choices=["both", "fp64", "mixed"], default="both",
                     help="with --cg-only: run one solver only (profiling)")
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
@@ -145,14 +146,16 @@ def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0):
 
 
 def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, tol=1e-10, seed=0,
-                 solvers=("fp64", "mixed_fp16"), repeats=3):
+                 solvers=("fp64", "mixed_fp16"), repeats=3, csw=None):
     """Even-odd preconditioned MRHS CG on M^+M^ to a true relative residual
     tol: pure FP64 vs mixed precision (FP16 dslash storage, FP32 Krylov
     vectors, FP64 reductions and reliable updates), same GPU, same inputs.
     Sources: RHS 0..min(k,4N)-1 unit deltas (spin-colour component r) at
     seeded even sites, the rest Gaussian noise. Each solver: one untimed
     2-iteration solve (tuning, work vectors), then `repeats` timed full solves
-    from x = 0; seconds = the fastest, seconds_all = every one."""
+    from x = 0; seconds = the fastest, seconds_all = every one. With csw the
+    operator is Wilson-clover, M^ = A_ee - kappa^2 D_eo A_oo^-1 D_oe with
+    A = 1 - (kappa csw / 2) sigma.F built from the same links."""
     import numpy as np
 
     from mrhs_qcu_amd import fields
@@ -165,7 +168,21 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
     gu = torch.view_as_real(fields.gauge_field_device(L, N, seed=seed, device=dev)).reshape(-1)
     gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, _lib.DOUBLE) // 8, dtype=torch.float64, device=dev)
     _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), _lib.DOUBLE, gu.data_ptr(), _lib.DOUBLE, geom, sp))
+    cl = cli = None
+    if csw is not None:
+        nc = lib.qcuMrhsCloverBytes(geom, _lib.DOUBLE) // 8
+        cl = torch.empty(nc, dtype=torch.float64, device=dev)
+        cli = torch.empty(nc, dtype=torch.float64, device=dev)
+        _lib.check(lib.qcuMrhsCloverCompute(cl.data_ptr(), cli.data_ptr(), _lib.DOUBLE, gu.data_ptr(), _lib.DOUBLE,
+                                            -kappa * csw / 2, geom, sp))
     del gu
+
+    def solve(x, prm, res):
+        if cl is None:
+            return lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp)
+        return lib.qcuMrhsCgSolveClover(x.data_ptr(), b.data_ptr(), gn.data_ptr(), cl.data_ptr(), cli.data_ptr(), k,
+                                        geom, ctypes.byref(prm), res, sp)
+
     b = torch.zeros(S, 4, N, k, 2, dtype=torch.float64, device=dev)
     npt = min(k, 4 * N)
     rng = np.random.Generator(np.random.PCG64(seed + 11))
@@ -176,7 +193,8 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
         gen = torch.Generator(device=dev)
         gen.manual_seed(seed + 12)
         b[:, :, :, npt:, :] = torch.randn(S, 4, N, k - npt, 2, generator=gen, device=dev, dtype=torch.float64)
-    out = {"workload": f"su{N}_eo_cg_mdagm_{'x'.join(map(str, L))}_nrhs{k}_kappa{kappa}", "tol": tol,
+    op = "cg_mdagm" if csw is None else f"clover_csw{csw}_cg_mdagm"
+    out = {"workload": f"su{N}_eo_{op}_{'x'.join(map(str, L))}_nrhs{k}_kappa{kappa}", "tol": tol,
            "sources": f"{npt} point sources + {k - npt} Gaussian"}
     sols = {}
     for name, sloppy in (("fp64", _lib.DOUBLE), ("mixed_fp16", _lib.HALF)):
@@ -188,7 +206,7 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
         # Untimed warm-up: tunes the operator tiles and allocates the solver's
         # work vectors (both kept for the timed solve).
         prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 2, 50, sloppy
-        _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res, sp))
+        _lib.check(solve(x, prm, res))
         # Timed: the full solve from x = 0, `repeats` times; the fastest is
         # reported (every repeat does the same work and must converge).
         walls = []
@@ -197,8 +215,7 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
             prm.kappa, prm.tol, prm.max_iter, prm.reliable_interval, prm.sloppy_prec = kappa, tol, 20000, 50, sloppy
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _lib.check(lib.qcuMrhsCgSolve(x.data_ptr(), b.data_ptr(), gn.data_ptr(), k, geom, ctypes.byref(prm), res,
-                                          sp))
+            _lib.check(solve(x, prm, res))
             torch.cuda.synchronize()
             walls.append(time.perf_counter() - t0)
             if not max(res) <= tol:
@@ -476,7 +493,8 @@ def main():
         return
     if args.cg_only:
         solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",)}[args.cg_solver]
-        print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), solvers=solvers)), flush=True)
+        print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), solvers=solvers, csw=args.cg_clover)),
+              flush=True)
         return
     pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE}[args.precision]
     pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8}[args.precision]
@@ -539,6 +557,7 @@ def main():
         line["extra"] = extra
         if not args.no_cg:
             line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))
+            line["clover_cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), repeats=1, csw=1.0)
     if not args.no_cpu and not args.profile:
         line["cpu_baseline"] = cpu_baseline(L, N, k)
     print(json.dumps(line), flush=True)
