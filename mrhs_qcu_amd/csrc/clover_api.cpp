@@ -59,11 +59,18 @@ bool sigma_blocks(SigmaBlocks *sb)
   return true;
 }
 
+// Device scratch of one synchronous call; the stream is drained before the
+// memory is freed, so an early error return cannot free what a launched
+// kernel still uses.
 struct DeviceBuf {
   void *p = nullptr;
+  hipStream_t st = nullptr;
   ~DeviceBuf()
   {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(p);
+    }
   }
 };
 
@@ -98,6 +105,7 @@ int qcuMrhsCloverCompute(void *clover, void *clover_inv, int prec, const void *g
   const long long S = sites_per_parity(g);
   const long long n = 2 * S * 8LL * g.N * g.N;  // complex numbers per field
   DeviceBuf a64, inv64;
+  a64.st = inv64.st = st;
   const bool direct = prec == PREC_DOUBLE;
   void *a = direct ? clover : nullptr;
   if (!direct) {

@@ -14,11 +14,18 @@ namespace {
 // Distance from SU(3) a link may have and still be stored as two rows.
 constexpr double kSu3Tol = 1e-5;
 
+// Device scratch of one synchronous call; the stream is drained before the
+// memory is freed, so an early error return cannot free what a launched
+// kernel still uses.
 struct DeviceScratch {
   void *p = nullptr;
+  hipStream_t st = nullptr;
   ~DeviceScratch()
   {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(p);
+    }
   }
 };
 
@@ -50,6 +57,7 @@ int qcuMrhsPackGauge12(void *dst, int dst_prec, const void *src, int src_prec, c
   hipStream_t st = static_cast<hipStream_t>(stream);
   const size_t native_bytes = size_t(sites_per_parity(g)) * 2 * 8 * 9 * 16;
   DeviceScratch native, bad;
+  native.st = bad.st = st;
   if (hipMalloc(&native.p, native_bytes) != hipSuccess || hipMalloc(&bad.p, sizeof(int)) != hipSuccess)
     return api_fail("out of device memory for the gauge compression");
   if (pack_gauge(native.p, PREC_DOUBLE, src, src_prec, 3, g.L, st, &err)) return api_fail(err);
