@@ -159,8 +159,8 @@ __global__ void __launch_bounds__(kBlasBlock) zero_kernel(C *__restrict__ v, lon
 // out[r] = sum_b partial[b][r] in two passes. Pass 1: G workgroups walk the
 // flat [nb][k] array with coalesced loads; the thread count per workgroup is
 // a multiple of k, so every thread keeps one RHS; each workgroup folds its
-// threads per RHS in a fixed order into stage[G][k]. Pass 2: one thread per
-// RHS sums the G stage values in order. Deterministic, no atomics.
+// threads per RHS in a fixed order into stage[G][k]. Pass 2: one workgroup
+// per RHS sums the G stage values. Deterministic, no atomics.
 __global__ void __launch_bounds__(256) reduce_pass1(const double *__restrict__ partial, long long n, int k,
                                                     double *__restrict__ stage)
 {
@@ -177,14 +177,20 @@ __global__ void __launch_bounds__(256) reduce_pass1(const double *__restrict__ p
   }
 }
 
+// One workgroup per RHS: the G stage values are loaded in parallel and
+// folded by a fixed-order tree.
 __global__ void __launch_bounds__(256) reduce_pass2(const double *__restrict__ stage, int G, int k,
                                                     double *__restrict__ out)
 {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= k) return;
-  double sum = 0.0;
-  for (int b = 0; b < G; ++b) sum += stage[size_t(b) * k + r];
-  out[r] = sum;
+  __shared__ double red[256];
+  const int r = blockIdx.x;
+  red[threadIdx.x] = (int)threadIdx.x < G ? stage[size_t(threadIdx.x) * k + r] : 0.0;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = red[0];
 }
 
 // alpha = rr / pAp for live RHS; pnorm = sqrt(sum |p|^2).
@@ -382,7 +388,7 @@ hipError_t blas_reduce(const double *partial, long long nb, int k, double *out, 
   if (G > kReduceStages) G = kReduceStages;
   if (G < 1) G = 1;
   hipLaunchKernelGGL(reduce_pass1, dim3(unsigned(G)), dim3(block), 0, st, partial, n, k, stage);
-  hipLaunchKernelGGL(reduce_pass2, dim3(unsigned((k + 255) / 256)), dim3(256), 0, st, stage, int(G), k, out);
+  hipLaunchKernelGGL(reduce_pass2, dim3(unsigned(k)), dim3(kReduceStages), 0, st, stage, int(G), k, out);
   return hipGetLastError();
 }
 
