@@ -422,3 +422,19 @@ def test_dslash_ex_scales_xpay_norms(lib, dev, in_prec, out_prec):
     assert err < max(tol[in_prec], tol[out_prec])
     wn = np.array([np.vdot(want[r], want[r]).real for r in range(k)])
     assert np.allclose(nrm.cpu().numpy(), wn, rtol=max(tol[in_prec], tol[out_prec]) * 4)
+
+
+@pytest.mark.parametrize("L", [(2, 2, 2, 2), (2, 4, 2, 6), (6, 2, 4, 2), (10, 6, 2, 4)])
+def test_extreme_lattice_shapes(lib, dev, L):
+    """Smallest and ragged extents (x half-extent 1, 3, 5; extents of 2 in
+    y, z or t, where both neighbours in a direction are the same site) with
+    the default tile and with every dimension split (boundary skip): FP64
+    against the oracle, both parities and dagger signs."""
+    N, k = 3, 3
+    g = fields.gauge_field(L, N, seed=31)
+    psi = fields.gaussian_spinors(L, N, k, seed=32)
+    for grid in ((1, 1, 1, 1), (2, 2, 2, 2)):
+        for parity, dagger in ((0, 0), (1, 1)):
+            got = run_packed(lib, dev, g, psi, L, N, parity, dagger, _lib.DOUBLE, grid=grid)
+            want = oracle.dslash(g, psi, L, parity, dagger, grid=grid)
+            assert rel_err(got, want) < TOL[2], (L, grid, parity, dagger)
