@@ -13,6 +13,7 @@ struct CgState {
   double *rr, *rr_new, *pap, *alpha, *beta, *pnorm2, *pnorm, *b2, *rdotp, *gamma;
   int *done, *sloppy_conv;
   float *sc_p, *sc_t, *sc_q, *sc_t2;  // sloppy storage scales (true = stored * sc)
+  float *sc_p_old;                    // scale of the stored p before cg_beta replaced sc_p
 };
 
 // Workgroups of every vector kernel for n complex numbers and k RHS (the
@@ -20,19 +21,22 @@ struct CgState {
 long long blas_num_blocks(long long n, int k);
 
 // x += alpha p (x = NULL: skipped) ; p = gamma r + beta p (gamma = NULL: 1) ;
-// ps = p / sc_p ; partial |p|^2
+// ps = p / sc_p ; partial |p|^2. p = NULL: the direction is kept only in ps,
+// the old one read as ps * sc_old (sc_old = NULL: 1).
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const double *gamma, void *x, const double *alpha, const float *sc_p, double *partial,
-                         long long n, int k, hipStream_t st);
+                         const double *gamma, void *x, const double *alpha, const float *sc_p, const float *sc_old,
+                         double *partial, long long n, int k, hipStream_t st);
 hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int k, hipStream_t st);
 // r = b - Ax (b FP64, Ax FP64 or, with FP32 r, FP32; difference in FP64, r
-// stored in prec_k; Ax may be r itself), partials |r|^2 and Re <r, p>.
-hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p,
-                         double *partial_rr, double *partial_rp, long long n, int k, hipStream_t st);
+// stored in prec_k; Ax may be r itself), partials |r|^2 and Re <r, p> with
+// the true p = stored p (prec_p) * sc_p (NULL: 1).
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p, int prec_p,
+                         const float *sc_p, double *partial_rr, double *partial_rp, long long n, int k,
+                         hipStream_t st);
 // xh (FP64) += x + alpha p, then x = 0 (x = NULL: xh += alpha p; p = NULL: no
-// alpha p term).
-hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, const double *alpha, long long n, int k,
-                           hipStream_t st);
+// alpha p term); p stored in prec_p, true p = stored * sc_p (NULL: 1).
+hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, int prec_p, const float *sc_p,
+                           const double *alpha, long long n, int k, hipStream_t st);
 hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st);
 // out[r] = sum over nb workgroups of partial[b][r]; stage: device scratch of
 // kReduceStages * k doubles.

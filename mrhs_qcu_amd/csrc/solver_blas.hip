@@ -31,14 +31,16 @@ __device__ __forceinline__ void block_partial(double v, int rhs, int k, double *
 }
 
 // x += alpha p (x null: skipped) ; p = gamma r + beta p (gamma null: 1) ;
-// p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2
+// p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2. With p null the
+// search direction lives only in p_s: the old p is p_s * sc_old (null: 1).
 template <class CK, class CS>
 __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p, CS *__restrict__ ps,
                                                               const CK *__restrict__ r,
                                                               const double *__restrict__ beta,
                                                               const double *__restrict__ gamma,
                                                               CK *__restrict__ x, const double *__restrict__ alpha,
-                                                              const float *__restrict__ sc_p, double *partial,
+                                                              const float *__restrict__ sc_p,
+                                                              const float *__restrict__ sc_old, double *partial,
                                                               long long n, int k)
 {
   using A = typename Acc<CK>::T;
@@ -49,9 +51,18 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
   const A c = gamma ? A(gamma[rhs]) : A(1);
   const A al = x ? A(alpha[rhs]) : A(0);
   const float inv = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
+  const A so = sc_old ? A(sc_old[rhs]) : A(1);
   double acc = 0.0;
   for (long long e = g; e < n; e += T) {
-    cpx<A> pp = ld_cvt<A>(p[e]), rr = ld_cvt<A>(r[e]);
+    cpx<A> pp;
+    if (p) {
+      pp = ld_cvt<A>(p[e]);
+    } else {
+      pp = ld_cvt<A>(ps[e]);
+      pp.re *= so;
+      pp.im *= so;
+    }
+    const cpx<A> rr = ld_cvt<A>(r[e]);
     if (x) {
       cpx<A> xx = ld_cvt<A>(x[e]);
       xx.re += al * pp.re;
@@ -60,7 +71,7 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
     }
     pp.re = c * rr.re + b * pp.re;
     pp.im = c * rr.im + b * pp.im;
-    p[e] = StCvt<CK>::go(pp);
+    if (p) p[e] = StCvt<CK>::go(pp);
     if (ps) ps[e] = StCvt<CS>::go(cpx<float>{float(pp.re) * inv, float(pp.im) * inv});
     acc += double(pp.re) * pp.re + double(pp.im) * pp.im;
   }
@@ -87,16 +98,17 @@ __global__ void __launch_bounds__(kBlasBlock) norm2_kernel(const C *__restrict__
 // r = b - Ax computed in FP64 (Ax stored as CA) and stored in the Krylov
 // precision CK; partials |r|^2 (of the FP64 values) and Re <r, p> (of the
 // stored r). Ax may be r itself (each element is read before it is written).
-template <class CK, class CA>
+template <class CK, class CA, class CP>
 __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *r, const c64 *__restrict__ b,
-                                                              const CA *Ax, const CK *__restrict__ p,
-                                                              double *partial_rr, double *partial_rp, long long n,
-                                                              int k)
+                                                              const CA *Ax, const CP *__restrict__ p,
+                                                              const float *__restrict__ sc_p, double *partial_rr,
+                                                              double *partial_rp, long long n, int k)
 {
   using A = typename Acc<CK>::T;
   const long long T = (long long)gridDim.x * blockDim.x;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int rhs = int(g % k);
+  const double sp = sc_p ? double(sc_p[rhs]) : 1.0;
   double acc = 0.0, dot = 0.0;
   for (long long e = g; e < n; e += T) {
     const c64 bb = b[e];
@@ -105,9 +117,11 @@ __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *r, const c64 *
     const CK rs = StCvt<CK>::go(rr);
     r[e] = rs;
     acc += rr.re * rr.re + rr.im * rr.im;
-    const cpx<A> rv = ld_cvt<A>(rs), pv = ld_cvt<A>(p[e]);
+    const cpx<A> rv = ld_cvt<A>(rs);
+    const cpx<A> pv = ld_cvt<A>(p[e]);
     dot += double(rv.re) * pv.re + double(rv.im) * pv.im;
   }
+  dot *= sp;
   block_partial(acc, rhs, k, partial_rr);
   __syncthreads();
   block_partial(dot, rhs, k, partial_rp);
@@ -115,15 +129,18 @@ __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *r, const c64 *
 
 // xh += x + alpha p (x, p nullable; the sum formed in x's precision, then
 // converted to FP64), then x = 0
-template <class CK>
+template <class CK, class CP>
 __global__ void __launch_bounds__(kBlasBlock) accumulate_kernel(c64 *__restrict__ xh, CK *__restrict__ x,
-                                                                const CK *__restrict__ p,
+                                                                const CP *__restrict__ p,
+                                                                const float *__restrict__ sc_p,
                                                                 const double *__restrict__ alpha, long long n, int k)
 {
   const long long T = (long long)gridDim.x * blockDim.x;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   using A = typename Acc<CK>::T;
-  const A al = p ? A(alpha[int(g % k)]) : A(0);
+  const int rhs = int(g % k);
+  // alpha times the true p (stored p times its scale)
+  const A al = p ? A(alpha[rhs]) * (sc_p ? A(sc_p[rhs]) : A(1)) : A(0);
   for (long long e = g; e < n; e += T) {
     cpx<double> v{0.0, 0.0};
     if (x) {
@@ -218,6 +235,7 @@ __global__ void beta_kernel(CgState s, int k, double stop2)
   const double bound = sqrt(rn) + fabs(beta) * s.pnorm[r];
   float sp = float(bound / 1024.0);
   if (!(sp > 1e-30f)) sp = 1e-30f;
+  s.sc_p_old[r] = s.sc_p[r];
   s.sc_p[r] = sp;
   s.sc_t[r] = 16.0f * sp;
   s.sc_q[r] = 4.0f * sp;
@@ -241,6 +259,7 @@ __global__ void init_state_kernel(CgState s, int k)
   float sp = float(sqrt(b2) / 1024.0);
   if (!(sp > 1e-30f)) sp = 1e-30f;
   s.sc_p[r] = sp;
+  s.sc_p_old[r] = sp;
   s.sc_t[r] = 16.0f * sp;
   s.sc_q[r] = 4.0f * sp;
   s.sc_t2[r] = 64.0f * sp;
@@ -298,28 +317,23 @@ long long blas_num_blocks(long long n, int k) { return grid_for(n, k); }
 #define QM_GRID(n, k) dim3(grid_for(n, k)), dim3(kBlasBlock), 0, st
 
 hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const double *gamma, void *x, const double *alpha, const float *sc_p, double *partial,
-                         long long n, int k, hipStream_t st)
+                         const double *gamma, void *x, const double *alpha, const float *sc_p, const float *sc_old,
+                         double *partial, long long n, int k, hipStream_t st)
 {
+#define QM_UP(CK, CS)                                                                                          \
+  hipLaunchKernelGGL((update_p_kernel<CK, CS>), QM_GRID(n, k), static_cast<CK *>(p), static_cast<CS *>(ps),    \
+                     static_cast<const CK *>(r), beta, gamma, static_cast<CK *>(x), alpha, sc_p, sc_old, partial, \
+                     n, k)
   if (prec_k == PREC_DOUBLE) {
-    hipLaunchKernelGGL((update_p_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(p), static_cast<c64 *>(ps),
-                       static_cast<const c64 *>(r), beta, gamma, static_cast<c64 *>(x), alpha, sc_p, partial, n, k);
+    QM_UP(c64, c64);
   } else {
     switch (prec_s) {
-      case PREC_HALF:
-        hipLaunchKernelGGL((update_p_kernel<c32, c16>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c16 *>(ps),
-                           static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x), alpha, sc_p, partial, n, k);
-        break;
-      case PREC_BF16:
-        hipLaunchKernelGGL((update_p_kernel<c32, cb16>), QM_GRID(n, k), static_cast<c32 *>(p),
-                           static_cast<cb16 *>(ps), static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x),
-                           alpha, sc_p, partial, n, k);
-        break;
-      default:
-        hipLaunchKernelGGL((update_p_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(p), static_cast<c32 *>(ps),
-                           static_cast<const c32 *>(r), beta, gamma, static_cast<c32 *>(x), alpha, sc_p, partial, n, k);
+      case PREC_HALF: QM_UP(c32, c16); break;
+      case PREC_BF16: QM_UP(c32, cb16); break;
+      default: QM_UP(c32, c32);
     }
   }
+#undef QM_UP
   return hipGetLastError();
 }
 
@@ -341,30 +355,53 @@ hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int
   return hipGetLastError();
 }
 
-hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p,
-                         double *partial_rr, double *partial_rp, long long n, int k, hipStream_t st)
+hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int prec_ax, const void *p, int prec_p,
+                         const float *sc_p, double *partial_rr, double *partial_rp, long long n, int k,
+                         hipStream_t st)
 {
-  if (prec_k == PREC_DOUBLE)
-    hipLaunchKernelGGL((residual_kernel<c64, c64>), QM_GRID(n, k), static_cast<c64 *>(r), static_cast<const c64 *>(b),
-                       static_cast<const c64 *>(Ax), static_cast<const c64 *>(p), partial_rr, partial_rp, n, k);
-  else if (prec_ax == PREC_SINGLE)
-    hipLaunchKernelGGL((residual_kernel<c32, c32>), QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
-                       static_cast<const c32 *>(Ax), static_cast<const c32 *>(p), partial_rr, partial_rp, n, k);
-  else
-    hipLaunchKernelGGL((residual_kernel<c32, c64>), QM_GRID(n, k), static_cast<c32 *>(r), static_cast<const c64 *>(b),
-                       static_cast<const c64 *>(Ax), static_cast<const c32 *>(p), partial_rr, partial_rp, n, k);
+#define QM_RES(CK, CA, CP)                                                                                       \
+  hipLaunchKernelGGL((residual_kernel<CK, CA, CP>), QM_GRID(n, k), static_cast<CK *>(r),                          \
+                     static_cast<const c64 *>(b), static_cast<const CA *>(Ax), static_cast<const CP *>(p), sc_p, \
+                     partial_rr, partial_rp, n, k)
+#define QM_RES_P(CK, CA)                         \
+  switch (prec_p) {                              \
+    case PREC_DOUBLE: QM_RES(CK, CA, c64); break; \
+    case PREC_SINGLE: QM_RES(CK, CA, c32); break; \
+    case PREC_HALF: QM_RES(CK, CA, c16); break;   \
+    default: QM_RES(CK, CA, cb16);                \
+  }
+  if (prec_k == PREC_DOUBLE) {
+    QM_RES_P(c64, c64)
+  } else if (prec_ax == PREC_SINGLE) {
+    QM_RES_P(c32, c32)
+  } else {
+    QM_RES_P(c32, c64)
+  }
+#undef QM_RES_P
+#undef QM_RES
   return hipGetLastError();
 }
 
-hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, const double *alpha, long long n, int k,
-                           hipStream_t st)
+hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, int prec_p, const float *sc_p,
+                           const double *alpha, long long n, int k, hipStream_t st)
 {
-  if (prec_k == PREC_DOUBLE)
-    hipLaunchKernelGGL(accumulate_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c64 *>(x),
-                       static_cast<const c64 *>(p), alpha, n, k);
-  else
-    hipLaunchKernelGGL(accumulate_kernel<c32>, QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<c32 *>(x),
-                       static_cast<const c32 *>(p), alpha, n, k);
+#define QM_ACC(CK, CP)                                                                                            \
+  hipLaunchKernelGGL((accumulate_kernel<CK, CP>), QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<CK *>(x), \
+                     static_cast<const CP *>(p), sc_p, alpha, n, k)
+#define QM_ACC_P(CK)                           \
+  switch (prec_p) {                            \
+    case PREC_DOUBLE: QM_ACC(CK, c64); break;   \
+    case PREC_SINGLE: QM_ACC(CK, c32); break;   \
+    case PREC_HALF: QM_ACC(CK, c16); break;     \
+    default: QM_ACC(CK, cb16);                  \
+  }
+  if (prec_k == PREC_DOUBLE) {
+    QM_ACC_P(c64)
+  } else {
+    QM_ACC_P(c32)
+  }
+#undef QM_ACC_P
+#undef QM_ACC
   return hipGetLastError();
 }
 

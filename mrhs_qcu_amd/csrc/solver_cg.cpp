@@ -266,7 +266,10 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   void *t64 = mem.bytes(b64, err, &ok);
   void *ax64 = mem.bytes(b64, err, &ok);
   void *r = mem.bytes(bk, err, &ok);
-  void *p = mem.bytes(bk, err, &ok);
+  // The search direction: FP64 in the FP64 solve; in a mixed solve it lives
+  // only in its (scaled) sloppy copy p_s, which the operator reads, so x and r
+  // advance along exactly the direction the operator was applied to.
+  void *p = mixed ? nullptr : mem.bytes(bk, err, &ok);
   void *xk = mixed ? mem.bytes(bk, err, &ok) : x_out;
   void *p_s = mixed ? mem.bytes(bs, err, &ok) : p;
   void *t_s = mixed ? mem.bytes(bs, err, &ok) : t64;
@@ -298,6 +301,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   cs.sc_t = mem.alloc<float>(k, err, &ok);
   cs.sc_q = mem.alloc<float>(k, err, &ok);
   cs.sc_t2 = mem.alloc<float>(k, err, &ok);
+  cs.sc_p_old = mem.alloc<float>(k, err, &ok);
   double *rr_true = mem.alloc<double>(k, err, &ok);
   // Clover: FP64 scratch for M^ in FP64 (also the loop's fields when the
   // solve is all FP64), sloppy copies of A and A^-1 and two sloppy fields.
@@ -358,6 +362,11 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   const bool scaled = mixed && ps == PREC_HALF;
   const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
   const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
+  const float *sc_p_old = scaled ? cs.sc_p_old : nullptr;
+  // The stored search direction, its precision and scale (as read by the
+  // accumulation and the residual of a reliable update, before cg_beta).
+  void *pdir = mixed ? p_s : p;
+  const int pdir_prec = mixed ? ps : pk;
 
 #define QM_TRY(expr)                                     \
   do {                                                   \
@@ -379,9 +388,9 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   } else {
     QM_HIP(hipMemcpyAsync(r, bhat, b64, hipMemcpyDeviceToDevice, st));
   }
-  QM_HIP(blas_zero(pk, p, n, k, st));
-  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, nullptr, nullptr, sc_p, partial_p, n,
-                       k, st));
+  QM_HIP(blas_zero(pdir_prec, pdir, n, k, st));
+  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, nullptr, nullptr, sc_p, sc_p_old,
+                       partial_p, n, k, st));
 
   std::vector<int> done(k, 0);
   std::vector<double> rrt(k, 0.0), b2h(k, 0.0), rr_ru(k, 0.0);
@@ -416,8 +425,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // just finished, when `step`), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces the
   // iterated norm, done flags; the Krylov residual restarts from r_h.
   auto reliable_update = [&](bool step, bool use32) -> bool {
-    if ((mixed || step) && !hip_ok(blas_accumulate(pk, x_out, mixed ? xk : nullptr, step ? p : nullptr, cs.alpha,
-                                                   n, k, st),
+    if ((mixed || step) && !hip_ok(blas_accumulate(pk, x_out, mixed ? xk : nullptr, step ? pdir : nullptr,
+                                                   pdir_prec, sc_p, cs.alpha, n, k, st),
                                    err, "accumulate"))
       return false;
     // r = b^ - M^+M^ x_h straight into the Krylov vector, with |r|^2 and
@@ -427,12 +436,15 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       if (convert_complex(x32, PREC_SINGLE, x_out, PREC_DOUBLE, n, st, &e2)) return *err = e2, false;
       if (!s.apply_m(rh32, x32, t32, g32, 0, PREC_SINGLE)) return false;
       if (!s.apply_m(r, rh32, t32, g32, 1, PREC_SINGLE)) return false;
-      if (!hip_ok(blas_residual(pk, r, bhat, r, PREC_SINGLE, p, s.partial, partial_p, n, k, st), err, "residual"))
+      if (!hip_ok(blas_residual(pk, r, bhat, r, PREC_SINGLE, pdir, pdir_prec, sc_p, s.partial, partial_p, n, k, st),
+                  err, "residual"))
         return false;
     } else {
       if (!s.apply_m64(rh, x_out, t64, gauge64, 0)) return false;
       if (!s.apply_m64(ax64, rh, t64, gauge64, 1)) return false;
-      if (!hip_ok(blas_residual(pk, r, bhat, ax64, PREC_DOUBLE, p, s.partial, partial_p, n, k, st), err, "residual"))
+      if (!hip_ok(blas_residual(pk, r, bhat, ax64, PREC_DOUBLE, pdir, pdir_prec, sc_p, s.partial, partial_p, n, k,
+                                st),
+                  err, "residual"))
         return false;
     }
     if (!hip_ok(blas_reduce(s.partial, nb_blas, k, rr_true, stage, st), err, "reduce")) return false;
@@ -518,13 +530,13 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       // made orthogonal to r_true.
       QM_HIP(cg_beta(cs, k, stop2, st));
       QM_HIP(cg_orth(cs, k, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, nullptr, nullptr, sc_p, partial_p,
-                           n, k, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, nullptr, nullptr, sc_p, sc_p_old,
+                           partial_p, n, k, st));
     } else {
       // x += alpha p ; beta = |r|^2 / |r_old|^2 ; p = r + beta p
       QM_HIP(cg_beta(cs, k, stop2, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, xk, cs.alpha, sc_p, partial_p, n,
-                           k, st));
+      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, xk, cs.alpha, sc_p, sc_p_old,
+                           partial_p, n, k, st));
     }
   }
   // Every iteration ends in a reliable update or in a p update that applied
