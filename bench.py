@@ -14,8 +14,13 @@ The JSON line also carries
                 with the tile it was measured on (`traffic_tile`);
   cpu_baseline: the FP32 C oracle (OpenMP) on the host cores, same workload,
                 a bounded number of applications;
-  extra:        FP16 / BF16 / FP64 storage at the same size and the CG
-                time-to-solution comparison when the solver is built.
+  extra:        FP16 / BF16 / FP64 storage and two-row SU(3) links at the
+                same size, a cache-resident 16^3x32 case, the batched graph,
+                one process's share of a split lattice (interior, face pack,
+                boundary terms), the clover term, and the qcu.h end-to-end
+                call;
+  cg, clover_cg: mixed-precision vs FP64 CG time to a 1e-10 true residual
+                at 32^3x64, nrhs = 16, Wilson and Wilson-clover.
 """
 import argparse
 import ctypes
