@@ -175,6 +175,55 @@ __device__ __forceinline__ void hop_one(cpx<A> (&acc)[4][N], const cpx<A> (&V)[N
   }
 }
 
+// The hop for all R right-hand sides of a lane with the link read from LDS
+// one row at a time inside the mat-vec (each row used by every RHS), for
+// large N, where a whole N x N link in registers does not fit beside the
+// accumulators and two neighbour spinors.
+template <int N, int D, int R, class A, class CG>
+__device__ __forceinline__ void hop_rows(cpx<A> (&acc)[R][4][N], const CG *vp, const cpx<A> (&psi)[R][4][N], A s)
+{
+  constexpr HopEntry e = hop_entry(D);
+  cpx<A> h0[R][N], h1[R][N];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      h0[r][c] = add_phase<e.k0>(psi[r][0][c], psi[r][e.a0][c], s);
+      h1[r][c] = add_phase<e.k1>(psi[r][1][c], psi[r][e.a1][c], s);
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    cpx<A> v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = ld_cvt<A>(vp[i * N + j]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cpx<A> g0{A(0), A(0)}, g1{A(0), A(0)};
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        g0.re = fmaA(v[j].re, h0[r][j].re, g0.re);
+        g0.im = fmaA(v[j].re, h0[r][j].im, g0.im);
+        g1.re = fmaA(v[j].re, h1[r][j].re, g1.re);
+        g1.im = fmaA(v[j].re, h1[r][j].im, g1.im);
+        g0.re = fmaA(-v[j].im, h0[r][j].im, g0.re);
+        g0.im = fmaA(v[j].im, h0[r][j].re, g0.im);
+        g1.re = fmaA(-v[j].im, h1[r][j].im, g1.re);
+        g1.im = fmaA(v[j].im, h1[r][j].re, g1.im);
+      }
+      acc[r][0][i].re += g0.re;
+      acc[r][0][i].im += g0.im;
+      acc[r][1][i].re += g1.re;
+      acc[r][1][i].im += g1.im;
+      acc[r][e.a0][i] = add_phase<ConjPhase<e.k0>::v>(acc[r][e.a0][i], g0, s);
+      acc[r][e.a1][i] = add_phase<ConjPhase<e.k1>::v>(acc[r][e.a1][i], g1, s);
+    }
+  }
+}
+
+// Large-N threshold of the row-wise link path (native links, from LDS when
+// staged, else from global memory; no link is prefetched into registers).
+constexpr int kRowLinkN = 5;
+
 // Site coordinates and neighbours of one output site.
 struct SiteGeo {
   int idx;       // checkerboard index ((t*Lz+z)*Ly+y)*subLx+xh
@@ -391,10 +440,11 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
                                       const CG *blk8, A s, cpx<A> (&psi_cur)[R][4][N], cpx<A> (&V_cur)[N][N],
                                       cpx<A> (&psi_nxt)[R][4][N], cpx<A> (&V_nxt)[N][N])
 {
+  constexpr bool ROWS = GL == GL_NATIVE && N >= kRowLinkN;
   if constexpr (D == 0) {
     // X- data: every lane reads its predecessor's X+ spinor, then the lanes
     // without a predecessor in the row overwrite it with their own load.
-    if constexpr (!LL) fetch_link<1, N, GL, CG, A>(V_nxt, a, g, blk8);
+    if constexpr (!LL && !ROWS) fetch_link<1, N, GL, CG, A>(V_nxt, a, g, blk8);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -405,12 +455,16 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
     if (g.xown) load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[1], slot);
   } else if constexpr (D + 1 < 8) {
     load_spinor<N, R, SL, CS, A>(psi_nxt, a, a.in, g.nbr[D + 1], slot);
-    if constexpr (!LL) fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
+    if constexpr (!LL && !ROWS) fetch_link<D + 1, N, GL, CG, A>(V_nxt, a, g, blk8);
   }
   if (g.live[D]) {
-    if constexpr (LL) fetch_link<D, N, GL, CG, A>(V_cur, a, g, blk8);
+    if constexpr (ROWS) {
+      hop_rows<N, D, R, A, CG>(acc, blk8 + native_link_slot(D) * N * N, psi_cur, s);
+    } else {
+      if constexpr (LL) fetch_link<D, N, GL, CG, A>(V_cur, a, g, blk8);
 #pragma unroll
-    for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V_cur, psi_cur[r], s);
+      for (int r = 0; r < R; ++r) hop_one<N, D, A>(acc[r], V_cur, psi_cur[r], s);
+    }
   }
   if constexpr (D + 1 < 8) {
     if constexpr (LL)
@@ -760,7 +814,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
       constexpr bool LL = LDS && (GL == GL_NATIVE || GL == GL_NATIVE12);
-      if constexpr (!LL) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
+      if constexpr (!LL && !(GL == GL_NATIVE && N >= kRowLinkN)) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
       sweep<0, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
     }
 
