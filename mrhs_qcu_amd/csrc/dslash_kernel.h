@@ -662,14 +662,59 @@ __device__ __forceinline__ void load_link_pg(typename Pair<C>::V (&V)[N][N], con
   }
 }
 
+// 16-bit hop for all R right-hand sides of a lane with the link read one row
+// at a time where it is used (large N; see hop_rows).
+template <class C, int N, int D, int R>
+__device__ __forceinline__ void hop_rows_p(cpx<float> (&acc)[R][4][N], const C *vp,
+                                           const typename Pair<C>::V (&psi)[R][4][N], float s)
+{
+  using P = Pair<C>;
+  constexpr HopEntry e = hop_entry(D);
+  typename P::V h0s[R][N], h1s[R][N];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      h0s[r][c] = P::template proj_swapped<e.k0>(psi[r][0][c], psi[r][e.a0][c], s);
+      h1s[r][c] = P::template proj_swapped<e.k1>(psi[r][1][c], psi[r][e.a1][c], s);
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    typename P::V v[N], vc[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      v[j] = P::load(vp[i * N + j]);
+      vc[j] = P::link_c(v[j]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cpx<float> g0{0.f, 0.f}, g1{0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        g0.re = P::dot(vc[j], h0s[r][j], g0.re);
+        g0.im = P::dot(v[j], h0s[r][j], g0.im);
+        g1.re = P::dot(vc[j], h1s[r][j], g1.re);
+        g1.im = P::dot(v[j], h1s[r][j], g1.im);
+      }
+      acc[r][0][i].re += g0.re;
+      acc[r][0][i].im += g0.im;
+      acc[r][1][i].re += g1.re;
+      acc[r][1][i].im += g1.im;
+      acc[r][e.a0][i] = add_phase<ConjPhase<e.k0>::v>(acc[r][e.a0][i], g0, s);
+      acc[r][e.a1][i] = add_phase<ConjPhase<e.k1>::v>(acc[r][e.a1][i], g1, s);
+    }
+  }
+}
+
 template <class C, int D, int N, int R, bool LL, int GL>
 __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
                                         const C *blk8, float s, typename Pair<C>::V (&psi_cur)[R][4][N],
                                         typename Pair<C>::V (&V_cur)[N][N], typename Pair<C>::V (&psi_nxt)[R][4][N],
                                         typename Pair<C>::V (&V_nxt)[N][N])
 {
+  constexpr bool ROWS = GL == GL_NATIVE && N >= kRowLinkN;
   if constexpr (D == 0) {
-    if constexpr (!LL) load_link_pg<C, N, GL>(V_nxt, blk8, 1, g);
+    if constexpr (!LL && !ROWS) load_link_pg<C, N, GL>(V_nxt, blk8, 1, g);
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -681,17 +726,21 @@ __device__ __forceinline__ void sweep_p(cpx<float> (&acc)[R][4][N], const Dslash
     if (g.xown) load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[1], slot);
   } else if constexpr (D + 1 < 8) {
     load_spinor_p<C, N, R>(psi_nxt, a, g.nbr[D + 1], slot);
-    if constexpr (!LL) load_link_pg<C, N, GL>(V_nxt, blk8, D + 1, g);
+    if constexpr (!LL && !ROWS) load_link_pg<C, N, GL>(V_nxt, blk8, D + 1, g);
   }
   if (g.live[D]) {
-    if constexpr (LL) load_link_pg<C, N, GL>(V_cur, blk8, D, g);
-    typename Pair<C>::V Vc[N][N];
+    if constexpr (ROWS) {
+      hop_rows_p<C, N, D, R>(acc, blk8 + native_link_slot(D) * N * N, psi_cur, s);
+    } else {
+      if constexpr (LL) load_link_pg<C, N, GL>(V_cur, blk8, D, g);
+      typename Pair<C>::V Vc[N][N];
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+      for (int i = 0; i < N; ++i)
 #pragma unroll
-      for (int j = 0; j < N; ++j) Vc[i][j] = Pair<C>::link_c(V_cur[i][j]);
+        for (int j = 0; j < N; ++j) Vc[i][j] = Pair<C>::link_c(V_cur[i][j]);
 #pragma unroll
-    for (int r = 0; r < R; ++r) hop_one_p<C, N, D>(acc[r], V_cur, Vc, psi_cur[r], s);
+      for (int r = 0; r < R; ++r) hop_one_p<C, N, D>(acc[r], V_cur, Vc, psi_cur[r], s);
+    }
   }
 
   if constexpr (D + 1 < 8) {
@@ -808,7 +857,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       using PV = typename Pair<CS>::V;
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor_p<CS, N, R>(psi0, a, g.nbr[0], slot);
-      if constexpr (!LDS) load_link_pg<CS, N, GL>(V0, blk8, 0, g);
+      if constexpr (!LDS && !(GL == GL_NATIVE && N >= kRowLinkN)) load_link_pg<CS, N, GL>(V0, blk8, 0, g);
       sweep_p<CS, 0, N, R, LDS, GL>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];

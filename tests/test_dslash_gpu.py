@@ -441,7 +441,7 @@ def test_extreme_lattice_shapes(lib, dev, L):
 
 
 @pytest.mark.parametrize("N", [5, 6, 8])
-@pytest.mark.parametrize("prec", [_lib.DOUBLE, _lib.SINGLE, _lib.HALF])
+@pytest.mark.parametrize("prec", [_lib.DOUBLE, _lib.SINGLE, _lib.HALF, _lib.BF16])
 def test_large_n_link_rows(lib, dev, N, prec):
     """N >= 5 reads each link row by row inside the mat-vec, from LDS or from
     global memory: both tile kinds (and R = 2 where instantiated) against the
@@ -449,7 +449,7 @@ def test_large_n_link_rows(lib, dev, N, prec):
     # k = 16 and 64-lane workgroups keep the LDS-staged links of a workgroup
     # (a few sites of 8 N^2 complex) within the per-workgroup LDS budget.
     L, k = (4, 4, 4, 8), 16
-    g = fields.gauge_field(L, N, seed=40 + N, smear_eps=0.3 if prec == _lib.HALF else None)
+    g = fields.gauge_field(L, N, seed=40 + N, smear_eps=0.3 if prec in (_lib.HALF, _lib.BF16) else None)
     psi = fields.gaussian_spinors(L, N, k, seed=41 + N)
     want = oracle.dslash(g, psi, L, 1, 0)
     tiles = [_lib.tile(R=1, block=64, lds=0), _lib.tile(R=1, block=64, lds=1)]
