@@ -206,3 +206,46 @@ def test_default_tiles_are_launchable_shapes(lib):
                         P = k // t.rhs_per_lane
                         shm = ((t.block - 1) // P + 2) * 8 * N * N * bytes_per_complex[prec]
                         assert shm <= lds_budget, (L, N, k, prec, t.as_tuple())
+
+
+TUNE_CHILD = r"""
+import ctypes, json, sys
+sys.path.insert(0, %(root)r)
+from mrhs_qcu_amd import _lib
+lib = _lib.load()
+out = []
+for L, k in (((8, 8, 8, 8), 4), ((8, 8, 8, 8), 6), ((4, 4, 4, 8), 4)):
+    t = _lib.Tile()
+    assert lib.qcuMrhsGetTile(_lib.geom(L, 3), _lib.SINGLE, k, ctypes.byref(t)) == 0
+    out.append(t.as_tuple())
+print(json.dumps(out))
+"""
+
+
+def test_tune_cache_file_is_keyed_by_architecture_and_shape(tmp_path):
+    """The autotuner's cache file (QCU_MRHS_TUNE_CACHE): lines for this
+    process's GPU architecture ("unknown" without a GPU) and shape are used,
+    later lines win, other architectures and shapes are ignored."""
+    import json
+
+    import torch
+
+    arch = "unknown"
+    if torch.cuda.is_available():
+        arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    path = tmp_path / "tune.txt"
+    path.write_text("gfx000 3 4 1 8 8 8 8 0 1 64 0 1 0 0 0 0\n"
+                    f"{arch} 3 4 1 8 8 8 8 0 1 256 0 2 2 0 0 1\n"
+                    f"{arch} 3 4 1 8 8 8 8 0 2 256 1 4 2 3 0 1\n"
+                    f"{arch} 3 6 1 8 8 8 8 0 2 128 1 2 4 0 0 1\n")
+    code = TUNE_CHILD % {"root": ROOT}
+    env = dict(os.environ, QCU_MRHS_TUNE_CACHE=str(path))
+    p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    got = json.loads(p.stdout.strip().splitlines()[-1])
+    assert tuple(got[0]) == (2, 256, 1, 4, 2, 3, 0, 1)
+    assert tuple(got[1]) == (2, 128, 1, 2, 4, 0, 0, 1)
+    env.pop("QCU_MRHS_TUNE_CACHE")
+    p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    default = json.loads(p.stdout.strip().splitlines()[-1])
+    assert tuple(got[2]) == tuple(default[2])  # shape not in the file: default tile
