@@ -48,7 +48,10 @@ typedef struct qcuMrhsTile_s {
   int t_block;      /* sites traversed in blocks of t_block time slices ... */
   int z_block;      /* ... and z_block z slices; (1, 0) = natural order, 0 = Lz */
   int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads;
-                       bit 2: keep the hardware workgroup order (no XCD-contiguous remap) */
+                       bit 2: keep the hardware workgroup order (no XCD-contiguous remap);
+                       bit 3 (traversal 1 only): XCD-cyclic order, the 8 XCDs work side by side
+                       on adjacent halves of y-streamed blocks (ignored when the lattice does
+                       not split into whole chunks of workgroups) */
   int occupancy;    /* cap on resident workgroups per CU (3..8), 0 = no cap */
   int traversal;    /* 0: each (t_block, z_block) block in (t, z, y, x) order;
                        1: the block streamed along y, (y, t, z, x) order inside it */

@@ -130,6 +130,7 @@ static void fill_common(DslashArgs &a, const DslashCall &c)
   a.t_flip = c.t_flip ? 1 : 0;
   a.rx_r = c.rx_r;
   a.rx_alpha = c.rx_alpha;
+  a.plain_out = (!c.x && c.xb == 1.0 && !c.sc_in && !c.sc_out && !c.partial && !c.rx_r) ? 1 : 0;
   a.sign = c.dagger ? -1.0f : 1.0f;
   a.parity = c.parity;
   a.subLx = c.geom.L[0] / 2;
@@ -160,6 +161,21 @@ long dslash_num_blocks(const DslashCall &c)
   const long lanes = S * (c.k / (R > 0 ? R : 1));
   const int B = c.tile.block > 0 ? c.tile.block : 256;
   return (lanes + B - 1) / B;
+}
+
+unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k)
+{
+  if (!(t.hints & 8) || (t.hints & 4) || !t.ystream || t.R < 1 || k % t.R) return 0;
+  const int B = t.block > 0 ? t.block : 256;
+  const long long tb = t.tblock < 1 ? 1 : t.tblock;
+  const long long zb = t.zblock < 1 ? g.L[2] : t.zblock;
+  const long long P = k / t.R;
+  const long long lanes = 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3] * P;
+  const long long half = tb * zb * (g.L[1] / 2) * (g.L[0] / 2) * P;  // lanes per half block
+  if (half % B || lanes % B) return 0;
+  const long long gran = half / B, nblocks = lanes / B;
+  if (nblocks % (8 * gran)) return 0;
+  return unsigned(gran);
 }
 
 int launch_dslash(const DslashCall &c, std::string *err)
@@ -207,6 +223,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
     a.nlanes = uint32_t(lanes);
     a.nblocks = uint32_t((lanes + B - 1) / B);
+    a.xcd_gran = xcd_cycle_gran(c.geom, c.tile, c.k);
     size_t shm = 0;
     if (lds) {
       const size_t nsite_max = size_t((B - 1) / a.P + 2);

@@ -71,7 +71,8 @@ struct Tile {
   int tblock = 1;  // traversal in blocks of tblock time slices ...
   int zblock = 0;  // ... and zblock z slices (0 = Lz: no z blocking)
   int hints = 0;   // bit 0: non-temporal output stores, bit 1: non-temporal link loads,
-                   // bit 2: keep the hardware workgroup order (no XCD remap)
+                   // bit 2: keep the hardware workgroup order (no XCD remap),
+                   // bit 3: XCD-cyclic order over half y-streamed blocks (xcd_cycle_gran)
   int occ = 0;     // cap on resident workgroups per CU (LDS padding); 0 = no cap
   int ystream = 0; // 1: stream each (tblock, zblock) block along y
 };
@@ -101,6 +102,14 @@ struct DslashCall {
 int launch_dslash(const DslashCall &c, std::string *err);
 // Workgroups a packed-layout launch uses (sizing of the partial-sum buffer).
 long dslash_num_blocks(const DslashCall &c);
+
+// Workgroups per chunk of the XCD-cyclic order (cache hint bit 3) for a
+// packed launch: half of one y-streamed (t_block x z_block) block, so the 8
+// XCDs work side by side on adjacent blocks and share their halo sites
+// through the memory-side cache. 0 when the tile does not ask for it or the
+// lattice does not split into whole chunks (the launch then keeps the
+// XCD-contiguous order).
+unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k);
 // Default tile for a packed launch (used when the autotuner has no entry).
 Tile default_tile(int N, int k, int prec);
 Tile default_tile(const Geom &g, int k, int prec);  // with a traversal for the lattice

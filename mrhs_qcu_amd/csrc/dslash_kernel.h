@@ -80,6 +80,8 @@ struct DslashArgs {
   int t_flip;   // GL_NATIVE12: the time links that cross the t boundary carry a factor -1 (antiperiodic
                 // time folded into the links), which two stored rows cannot express
   int aos_vec;  // pointer layouts: every buffer 16-byte aligned (16-byte loads and stores)
+  int plain_out;  // packed output is D(in) as it is: no xpay operand, scales, norms or fused update
+  uint32_t xcd_gran;  // > 0: XCD-cyclic workgroup order in chunks of xcd_gran workgroups (xcd_cyclic)
 };
 
 // Hop table: direction D = 2*mu + (backward); projection partner spins a0/a1
@@ -790,7 +792,8 @@ template <int N, int R, int SL, int GL, bool LDS, bool SITE_MINOR, class CS, cla
 __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t blk = a.xcd_map ? xcd_remap(blockIdx.x, a.nblocks) : blockIdx.x;
+  const uint32_t blk =
+      a.xcd_gran ? xcd_cyclic(blockIdx.x, a.xcd_gran) : (a.xcd_map ? xcd_remap(blockIdx.x, a.nblocks) : blockIdx.x);
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;
@@ -918,28 +921,41 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       const size_t stride = size_t(a.k) / R;
       CVec<CO, R> *po = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.out) + base);
       const CVec<CS, R> *px = a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
+      if (a.plain_out) {
+        // The epilogue runs after the wave's last load has returned, so its
+        // instructions are not hidden behind memory: store acc directly.
 #pragma unroll
-      for (int sc = 0; sc < 4 * N; ++sc) {
-        CVec<CO, R> o;
-        CVec<CS, R> xv;
-        if (px) xv = px[sc * stride];
+        for (int sc = 0; sc < 4 * N; ++sc) {
+          CVec<CO, R> o;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          cpx<A> v = acc[r][sc / N][sc % N];
-          v.re *= fin[r];
-          v.im *= fin[r];
-          if (px) {
-            const cpx<A> xx = ld_cvt<A>(xv.v[r]);
-            v.re += fx[r] * xx.re;
-            v.im += fx[r] * xx.im;
-          }
-          nrm[r] += v.re * v.re + v.im * v.im;
-          v.re *= fo[r];
-          v.im *= fo[r];
-          o.v[r] = StCvt<CO>::go(v);
+          for (int r = 0; r < R; ++r) o.v[r] = StCvt<CO>::go(acc[r][sc / N][sc % N]);
+          if (a.nt_store) store_nt(po + sc * stride, o);
+          else po[sc * stride] = o;
         }
-        if (a.nt_store) store_nt(po + sc * stride, o);
-        else po[sc * stride] = o;
+      } else {
+#pragma unroll
+        for (int sc = 0; sc < 4 * N; ++sc) {
+          CVec<CO, R> o;
+          CVec<CS, R> xv;
+          if (px) xv = px[sc * stride];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            cpx<A> v = acc[r][sc / N][sc % N];
+            v.re *= fin[r];
+            v.im *= fin[r];
+            if (px) {
+              const cpx<A> xx = ld_cvt<A>(xv.v[r]);
+              v.re += fx[r] * xx.re;
+              v.im += fx[r] * xx.im;
+            }
+            nrm[r] += v.re * v.re + v.im * v.im;
+            v.re *= fo[r];
+            v.im *= fo[r];
+            o.v[r] = StCvt<CO>::go(v);
+          }
+          if (a.nt_store) store_nt(po + sc * stride, o);
+          else po[sc * stride] = o;
+        }
       }
     } else {
       constexpr int C = 16 / int(sizeof(CO));

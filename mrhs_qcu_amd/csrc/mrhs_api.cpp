@@ -229,6 +229,9 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
             t.zblock = zb;
             t.ystream = 1;
             out.push_back(t);
+            // The same tile with the XCDs side by side on adjacent blocks.
+            t.hints = 8;
+            if (xcd_cycle_gran(g, t, k)) out.push_back(t);
           }
         }
       }

@@ -158,4 +158,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t hw_block, uint32_t nblock
   return xcd * q + (xcd < rem ? xcd : rem) + i;
 }
 
+// XCD-cyclic order: the logical workgroups are cut into chunks of `gran`,
+// and the XCDs take consecutive chunks side by side (XCD x works on chunk
+// 8 j + x while the others work on chunks 8 j .. 8 j + 7), each XCD in its
+// chunk's own order. Needs nblocks to be a multiple of kNumXcd * gran.
+__device__ __forceinline__ uint32_t xcd_cyclic(uint32_t hw_block, uint32_t gran)
+{
+  const uint32_t xcd = hw_block % kNumXcd, i = hw_block / kNumXcd;
+  return ((i / gran) * kNumXcd + xcd) * gran + i % gran;
+}
+
 }  // namespace qmrhs

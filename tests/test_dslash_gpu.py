@@ -459,3 +459,38 @@ def test_large_n_link_rows(lib, dev, N, prec):
         got = run_packed(lib, dev, g, psi, L, N, 1, 0, prec, tile=t)
         err = rel_err(got, want)
         assert err < TOL[prec], (t.as_tuple(), err)
+
+
+def test_xcd_cyclic_order(lib, dev):
+    """Cache hint bit 3 (XCDs side by side on adjacent halves of y-streamed
+    blocks) only reorders workgroups: the output equals the oracle at a small
+    size, and equals the XCD-contiguous order bit for bit at 24^3x48,
+    nrhs = 16, FP32 with the tiles the tuner offers there. A tile the lattice
+    does not split into whole chunks runs in the contiguous order."""
+    L, N, k = (8, 4, 6, 8), 3, 8
+    g = fields.gauge_field(L, N, seed=21)
+    psi = fields.gaussian_spinors(L, N, k, seed=22)
+    want = oracle.dslash(g, psi, L, 0, 1)
+    # (R, block, lds, t_block, z_block): (2, 64, 1, 2, 3) splits into chunks of 3 workgroups;
+    # (2, 256, 1, 2, 3) does not split (half a block is 192 lanes).
+    for R, block, lds, tb, zb in ((2, 64, 1, 2, 3), (2, 64, 0, 2, 3), (1, 128, 1, 2, 2), (2, 256, 1, 2, 3)):
+        got = run_packed(lib, dev, g, psi, L, N, 0, 1, _lib.SINGLE, tile=_lib.tile(R, block, lds, tb, zb, 8, 0, 1))
+        assert rel_err(got, want) < TOL[1], (R, block, lds, tb, zb)
+
+    L, k = (24, 24, 24, 48), 16
+    geom = _lib.geom(L, N)
+    S = fields.half_volume(L)
+    torch.manual_seed(1)
+    gu = torch.randn(4, 2, S, N, N, 2, dtype=torch.float64, device=dev)
+    gnat = empty_prec((lib.qcuMrhsGaugeBytes(geom, _lib.SINGLE) // 4,), _lib.SINGLE, dev)
+    _lib.check(lib.qcuMrhsPackGauge(gnat.data_ptr(), _lib.SINGLE, gu.data_ptr(), _lib.DOUBLE, geom, stream()))
+    pin = torch.randn(S, 4, N, k, 2, device=dev)
+    for tb, zb, block in ((6, 6, 128), (3, 6, 128), (4, 6, 256), (2, 6, 128)):
+        outs = []
+        for hints in (0, 8):
+            out = torch.full_like(pin, float("nan"))
+            _lib.check(lib.qcuMrhsDslash(out.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 1, 0, geom,
+                                         _lib.tile(2, block, 1, tb, zb, hints, 0, 1), stream()))
+            outs.append(out)
+        torch.cuda.synchronize()
+        assert bool(torch.equal(outs[0], outs[1])), (tb, zb, block)
