@@ -227,7 +227,8 @@ int launch_dslash(const DslashCall &c, std::string *err)
     size_t shm = 0;
     if (lds) {
       const size_t nsite_max = size_t((B - 1) / a.P + 2);
-      shm = nsite_max * 8 * link_cplx(c.gl, c.geom.N) * size_t(bytes_of(c.prec_g));
+      // Two-row links are staged as full links (rebuilt while staging).
+      shm = nsite_max * 8 * link_cplx(GL_NATIVE, c.geom.N) * size_t(bytes_of(c.prec_g));
       if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
     }
     if (c.tile.occ > 0) {

@@ -375,10 +375,10 @@ __device__ __forceinline__ void load_link_native(cpx<A> (&V)[N][N], const CG *bl
 // Compressed SU(3) link: rows 0 and 1 stored, row 2 = conj(row 0 x row 1)
 // (unitarity and det = 1); neg flips the sign of the whole link.
 template <int N, class CG, class A>
-__device__ __forceinline__ void load_link_native12(cpx<A> (&V)[N][N], const CG *blk8, int D, bool neg)
+__device__ __forceinline__ void load_link_native12(cpx<A> (&V)[N][N], const CG *blk8, int slot, bool neg)
 {
   static_assert(N == 3, "two-row links are SU(3) only");
-  const CG *p = blk8 + D * 2 * N;
+  const CG *p = blk8 + slot * 2 * N;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -808,23 +808,51 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
   }
 
   const CG *blk8 = nullptr;
-  constexpr int LC = link_cplx(GL, N);  // stored complex numbers per link
+  // Two-row links staged in LDS are rebuilt to full links while staging, once
+  // per site and link instead of once per lane that reads them: the sweep
+  // then sees the native layout (GLS).
+  constexpr int GLS = (LDS && GL == GL_NATIVE12) ? int(GL_NATIVE) : GL;
+  constexpr int LC = link_cplx(GLS, N);  // complex numbers per link as the sweep reads them
   if constexpr ((GL == GL_NATIVE || GL == GL_NATIVE12) && LDS) {
     static_assert(!SITE_MINOR, "LDS link staging needs site-major lanes");
     // Stage the 8 links of every site of this workgroup once; all k/R lanes of
     // a site then read them from LDS (broadcast) instead of from L1/L2.
-    constexpr int chunk16 = (8 * LC * int(sizeof(CG))) / 16;
     const uint32_t first = fdiv(lane0, a.div_P);
     uint32_t lastl = lane0 + blockDim.x - 1;
     if (lastl >= a.nlanes) lastl = a.nlanes - 1;
     const uint32_t nsite = fdiv(lastl, a.div_P) - first + 1;
-    u32x4 *dst = reinterpret_cast<u32x4 *>(smem);
-    const u32x4 *src = static_cast<const u32x4 *>(a.gauge);
-    for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
-      const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
-      const uint32_t phys = logical_to_phys(first + sloc, a);
-      const u32x4 *ps = src + (size_t(a.parity) * a.S + phys) * chunk16 + w;
-      dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
+    if constexpr (GL == GL_NATIVE) {
+      constexpr int chunk16 = (8 * LC * int(sizeof(CG))) / 16;
+      u32x4 *dst = reinterpret_cast<u32x4 *>(smem);
+      const u32x4 *src = static_cast<const u32x4 *>(a.gauge);
+      for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
+        const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
+        const uint32_t phys = logical_to_phys(first + sloc, a);
+        const u32x4 *ps = src + (size_t(a.parity) * a.S + phys) * chunk16 + w;
+        dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
+      }
+    } else {
+      // One thread per (site, link): read the two stored rows, rebuild row 2
+      // (and the t-boundary sign), store the full link in the native slot.
+      using W = typename std::conditional<std::is_same<CG, c64>::value, double, float>::type;
+      CG *dst = reinterpret_cast<CG *>(smem);
+      for (uint32_t i = threadIdx.x; i < nsite * 8; i += blockDim.x) {
+        const uint32_t sloc = i >> 3, d = i & 7u;
+        const uint32_t phys = logical_to_phys(first + sloc, a);
+        const CG *src = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + phys) * 8 * link_cplx(GL, N);
+        bool neg = false;
+        if (a.t_flip && (d == 3 || d == 7)) {
+          const uint32_t t = phys / (uint32_t(a.subLx) * a.Ly * a.Lz);
+          neg = d == 3 ? int(t) == a.Lt - 1 : t == 0;
+        }
+        cpx<W> V[N][N];
+        load_link_native12<N, CG, W>(V, src, int(d), neg);  // d is the link's slot in the site block
+        CG *q = dst + (size_t(sloc) * 8 + d) * LC;
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+#pragma unroll
+          for (int c = 0; c < N; ++c) q[r * N + c] = StCvt<CG>::go(V[r][c]);
+      }
     }
     __syncthreads();
     if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * LC;
@@ -861,13 +889,13 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor_p<CS, N, R>(psi0, a, g.nbr[0], slot);
       if constexpr (!LDS && !(GL == GL_NATIVE && N >= kRowLinkN)) load_link_pg<CS, N, GL>(V0, blk8, 0, g);
-      sweep_p<CS, 0, N, R, LDS, GL>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
+      sweep_p<CS, 0, N, R, LDS, GLS>(acc, a, g, slot, blk8, float(s), psi0, V0, psi1, V1);
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
       constexpr bool LL = LDS && (GL == GL_NATIVE || GL == GL_NATIVE12);
       if constexpr (!LL && !(GL == GL_NATIVE && N >= kRowLinkN)) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
-      sweep<0, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
+      sweep<0, N, R, SL, GLS, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
     }
 
     // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.
