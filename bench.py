@@ -14,8 +14,8 @@ The JSON line also carries
                 with the tile it was measured on (`traffic_tile`);
   cpu_baseline: the FP32 C oracle (OpenMP) on the host cores, same workload,
                 a bounded number of applications;
-  extra:        FP16 / BF16 / FP64 storage and two-row SU(3) links at the
-                same size, a cache-resident 16^3x32 case, the batched graph,
+  extra:        FP16 / BF16 / FP64 storage and two-row and eight-real SU(3)
+                links at the same size, a cache-resident 16^3x32 case, the batched graph,
                 one process's share of a split lattice (interior, face pack,
                 boundary terms), the clover term, and the qcu.h end-to-end
                 call;
@@ -116,9 +116,10 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     return ms, best.as_tuple()
 
 
-def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0):
-    """The same workload on two-row SU(3) links (qcuMrhsDslash12, the tile of
-    the native layout); returns ms_per_step."""
+def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0, reals=12):
+    """The same workload on compressed SU(3) links, two rows (reals = 12,
+    qcuMrhsDslash12) or eight reals (reals = 8, qcuMrhsDslash8), with the
+    tile of the native layout; returns ms_per_step."""
     from mrhs_qcu_amd import fields
 
     dev = torch.device("cuda:0")
@@ -131,20 +132,22 @@ def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0):
         _GAUGE_CACHE[key] = torch.view_as_real(fields.gauge_field_device(L, 3, seed=seed, device=dev)).reshape(-1)
     gu = _GAUGE_CACHE[key]
     dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
-    g12 = torch.empty(lib.qcuMrhsGauge12Bytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
+    nbytes, pack, run = ((lib.qcuMrhsGauge12Bytes, lib.qcuMrhsPackGauge12, lib.qcuMrhsDslash12) if reals == 12 else
+                         (lib.qcuMrhsGauge8Bytes, lib.qcuMrhsPackGauge8, lib.qcuMrhsDslash8))
+    g12 = torch.empty(nbytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
     # The generator folds antiperiodic time into the links: t_sign = -1.
-    _lib.check(lib.qcuMrhsPackGauge12(g12.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, -1, sp))
+    _lib.check(pack(g12.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, -1, sp))
     a = torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt)
     bufs = [a, torch.empty_like(a)]
     for i in range(warmup):
-        _lib.check(lib.qcuMrhsDslash12(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k,
-                                       i & 1, 0, -1, geom, None, sp))
+        _lib.check(run(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0, -1,
+                       geom, None, sp))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for i in range(steps):
-        lib.qcuMrhsDslash12(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0,
-                            -1, geom, None, sp)
+        run(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0, -1, geom, None,
+            sp)
     e1.record(st)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
@@ -540,9 +543,11 @@ def main():
             bb = bytes_alg(S, k, N, b, b, b)
             extra[name] = {"ms_per_step": ms2, "sites_rhs_per_s": S * k / (ms2 * 1e-3),
                            "hbm_frac": bb / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tile": t2}
-        for name, prec, b in (("two_row_links_fp32", _lib.SINGLE, 4), ("two_row_links_fp16", _lib.HALF, 2)):
-            ms12 = time_dslash12(lib, _lib, torch, geom, prec, k, max(20, args.steps // 4), 10, L)
-            bb12 = S * (k * 8 * N * 2 * b + 8 * 2 * N * 2 * b)
+        for name, prec, b, reals in (("two_row_links_fp32", _lib.SINGLE, 4, 12), ("two_row_links_fp16", _lib.HALF, 2, 12),
+                                     ("eight_real_links_fp32", _lib.SINGLE, 4, 8),
+                                     ("eight_real_links_fp16", _lib.HALF, 2, 8)):
+            ms12 = time_dslash12(lib, _lib, torch, geom, prec, k, max(20, args.steps // 4), 10, L, reals=reals)
+            bb12 = S * (k * 8 * N * 2 * b + 8 * reals * b)
             extra[name] = {"ms_per_step": ms12, "sites_rhs_per_s": S * k / (ms12 * 1e-3),
                            "hbm_frac": bb12 / (ms12 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                            "hbm_frac_vs_full_link_bytes": bytes_alg(S, k, N, b, b, b) / (ms12 * 1e-3) / 1e9 /

@@ -128,6 +128,24 @@ int qcuMrhsPackGauge12(void *dst, int dst_prec, const void *src_user, int src_pr
 int qcuMrhsDslash12(void *out, const void *in, const void *gauge12, int prec, int nrhs, int parity, int dagger,
                     int t_sign, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
 
+/* ---- eight-real SU(3) links -------------------------------------------------
+ * Each link as U01, U02, U10 and (arg U00, arg U20): [2][S][8][4] complex
+ * slots in prec, 8 of 18 reals. The moduli of U00 and U20 follow from unit
+ * rows and columns, the other four entries from unitarity and det = 1; the
+ * rebuild divides by |U01|^2 + |U02|^2 and runs once per site while the
+ * links are staged in LDS (the tile must have link_lds = 1). No counterpart
+ * in the reference, which loads full links.
+ * qcuMrhsPackGauge8 checks every link as qcuMrhsPackGauge12 does and also
+ * refuses links with |U01|^2 + |U02|^2 < 1e-6 (a unit gauge, for example:
+ * storage errors would grow by up to 1000x); it synchronises the stream.
+ * qcuMrhsDslash8 is qcuMrhsDslash on that layout, with the same t_sign; tile
+ * NULL takes the native layout's tile with link_lds = 1. */
+size_t qcuMrhsGauge8Bytes(const qcuMrhsGeom *g, int prec);
+int qcuMrhsPackGauge8(void *dst, int dst_prec, const void *src_user, int src_prec, const qcuMrhsGeom *g, int t_sign,
+                      void *stream);
+int qcuMrhsDslash8(void *out, const void *in, const void *gauge8, int prec, int nrhs, int parity, int dagger,
+                   int t_sign, const qcuMrhsGeom *g, const qcuMrhsTile *tile, void *stream);
+
 /* ---- clover term -----------------------------------------------------------
  * The operator DSLASH_CLOVER names (include/qcu_enum.h:12 of the reference,
  * which reserves 72 complex numbers per site for it,

@@ -56,6 +56,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
                       "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces",
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
+                      "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
                       "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
                       "qcuMrhsCgSolveClover"]
 
@@ -108,6 +109,9 @@ def _declare(lib):
         "qcuMrhsGauge12Bytes": (ctypes.c_size_t, [G, I]),
         "qcuMrhsPackGauge12": (I, [P, I, P, I, G, I, P]),
         "qcuMrhsDslash12": (I, [P, P, P, I, I, I, I, I, G, T, P]),
+        "qcuMrhsGauge8Bytes": (ctypes.c_size_t, [G, I]),
+        "qcuMrhsPackGauge8": (I, [P, I, P, I, G, I, P]),
+        "qcuMrhsDslash8": (I, [P, P, P, I, I, I, I, I, G, T, P]),
         "qcuMrhsCloverBytes": (ctypes.c_size_t, [G, I]),
         "qcuMrhsCloverCompute": (I, [P, P, I, P, I, D, G, P]),
         "qcuMrhsCloverApply": (I, [P, P, P, I, I, I, G, P]),

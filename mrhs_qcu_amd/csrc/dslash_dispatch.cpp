@@ -209,9 +209,12 @@ int launch_dslash(const DslashCall &c, std::string *err)
       if (!c.rx_alpha) return fail(err, "fused CG update: null alpha"), 1;
       if (uintptr_t(c.rx_r) % al) return fail(err, "packed fields must be 16-byte aligned"), 1;
     }
-    if (c.gl != GL_NATIVE && c.gl != GL_NATIVE12)
-      return fail(err, "packed spinors use the native (or two-row SU(3)) gauge layout"), 1;
+    if (c.gl != GL_NATIVE && c.gl != GL_NATIVE12 && c.gl != GL_NATIVE8)
+      return fail(err, "packed spinors use the native (or two-row / eight-real SU(3)) gauge layout"), 1;
     if (c.gl == GL_NATIVE12 && c.geom.N != 3) return fail(err, "two-row links need nColors = 3"), 1;
+    if (c.gl == GL_NATIVE8 && c.geom.N != 3) return fail(err, "eight-real links need nColors = 3"), 1;
+    if (c.gl == GL_NATIVE8 && !c.tile.lds)
+      return fail(err, "eight-real links are rebuilt while staging: the tile needs link_lds = 1"), 1;
     const int lds = c.tile.lds ? 1 : 0;
     KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
     LaunchFn fn = find_kernel(key);
@@ -227,7 +230,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     size_t shm = 0;
     if (lds) {
       const size_t nsite_max = size_t((B - 1) / a.P + 2);
-      // Two-row links are staged as full links (rebuilt while staging).
+      // Two-row and eight-real links are staged as full links (rebuilt while staging).
       shm = nsite_max * 8 * link_cplx(GL_NATIVE, c.geom.N) * size_t(bytes_of(c.prec_g));
       if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
     }

@@ -35,10 +35,17 @@ enum GaugeLayout : int {
   GL_REF_SOA = 1,  // reference SoA [4][2][N*N][S]; backward links daggered on the fly
   GL_NATIVE12 = 2, // SU(3) only: [2][S][8][2][3], the native links as their first two rows;
                    // row 2 = conj(row 0 x row 1) is rebuilt in registers
+  GL_NATIVE8 = 3,  // SU(3) only: [2][S][8][4], eight reals per link: U01, U02, U10 and
+                   // (arg U00, arg U20); rebuilt while the links are staged in LDS
 };
 
 // Stored complex numbers per link of a gauge layout.
-__host__ __device__ constexpr int link_cplx(int GL, int N) { return GL == GL_NATIVE12 ? 2 * N : N * N; }
+__host__ __device__ constexpr int link_cplx(int GL, int N)
+{
+  return GL == GL_NATIVE12 ? 2 * N : (GL == GL_NATIVE8 ? 4 : N * N);
+}
+// Layouts that store each site's 8 links as one block ([2][S][8][...]).
+__host__ __device__ constexpr bool site_blocked(int GL) { return GL == GL_NATIVE || GL == GL_NATIVE12 || GL == GL_NATIVE8; }
 
 constexpr int kMaxPtrs = 32;  // RHS per launch for the pointer layouts
 struct PtrPack {
@@ -391,6 +398,57 @@ __device__ __forceinline__ void load_link_native12(cpx<A> (&V)[N][N], const CG *
     const A im = fmaA(a1.re, b2.im, fmaA(a1.im, b2.re, fmaA(-a2.re, b1.im, -(a2.im * b1.re))));
     V[2][j] = cpx<A>{re, -im};
   }
+  if (neg) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) V[i][j] = cpx<A>{-V[i][j].re, -V[i][j].im};
+  }
+}
+
+// Eight-real SU(3) link (slot `slot` of a site block): a = row 0, b = row 1,
+// c = row 2 with a1, a2, b0 stored and the phases of a0 and c0. The moduli
+// follow from unit rows and columns, |a0|^2 = 1 - n with n = |a1|^2 + |a2|^2
+// and |c0|^2 = n - |b0|^2, and the rest from unitarity with det = 1:
+//   b1 = -(c0* a2* + a0* a1 b0) / n    b2 = (c0* a1* - a0* a2 b0) / n
+//   c1 = (b0* a2* - a0* a1 c0) / n     c2 = -(b0* a1* + a0* a2 c0) / n
+// (n > 0 is checked when the field is packed); neg flips the whole link.
+template <class A> __device__ __forceinline__ cpx<A> cmul(cpx<A> x, cpx<A> y)
+{
+  return cpx<A>{fmaA(x.re, y.re, -(x.im * y.im)), fmaA(x.re, y.im, x.im * y.re)};
+}
+template <class A> __device__ __forceinline__ cpx<A> cconj(cpx<A> x) { return cpx<A>{x.re, -x.im}; }
+template <class A> __device__ __forceinline__ void sin_cos(A x, A *sn, A *cs);
+template <> __device__ __forceinline__ void sin_cos<float>(float x, float *sn, float *cs) { sincosf(x, sn, cs); }
+template <> __device__ __forceinline__ void sin_cos<double>(double x, double *sn, double *cs) { sincos(x, sn, cs); }
+
+template <int N, class CG, class A>
+__device__ __forceinline__ void load_link_native8(cpx<A> (&V)[N][N], const CG *blk8, int slot, bool neg)
+{
+  static_assert(N == 3, "eight-real links are SU(3) only");
+  const CG *p = blk8 + slot * 4;
+  const cpx<A> a1 = ld_cvt<A>(p[0]), a2 = ld_cvt<A>(p[1]), b0 = ld_cvt<A>(p[2]), th = ld_cvt<A>(p[3]);
+  const A n = fmaA(a1.re, a1.re, fmaA(a1.im, a1.im, fmaA(a2.re, a2.re, a2.im * a2.im)));
+  const A nb0 = fmaA(b0.re, b0.re, b0.im * b0.im);
+  const A ra0 = sqrt(A(1) - n > A(0) ? A(1) - n : A(0));
+  const A rc0 = sqrt(n - nb0 > A(0) ? n - nb0 : A(0));
+  A sa, ca, sc, cc;
+  sin_cos<A>(th.re, &sa, &ca);
+  sin_cos<A>(th.im, &sc, &cc);
+  const cpx<A> a0{ra0 * ca, ra0 * sa}, c0{rc0 * cc, rc0 * sc};
+  const A inv = A(1) / n;
+  const cpx<A> a0c = cconj(a0), b0c = cconj(b0), c0c = cconj(c0), a1c = cconj(a1), a2c = cconj(a2);
+  const cpx<A> t1 = cmul(c0c, a2c), t2 = cmul(a0c, cmul(a1, b0)), t3 = cmul(c0c, a1c), t4 = cmul(a0c, cmul(a2, b0));
+  const cpx<A> t5 = cmul(b0c, a2c), t6 = cmul(a0c, cmul(a1, c0)), t7 = cmul(b0c, a1c), t8 = cmul(a0c, cmul(a2, c0));
+  V[0][0] = a0;
+  V[0][1] = a1;
+  V[0][2] = a2;
+  V[1][0] = b0;
+  V[1][1] = cpx<A>{-(t1.re + t2.re) * inv, -(t1.im + t2.im) * inv};
+  V[1][2] = cpx<A>{(t3.re - t4.re) * inv, (t3.im - t4.im) * inv};
+  V[2][0] = c0;
+  V[2][1] = cpx<A>{(t5.re - t6.re) * inv, (t5.im - t6.im) * inv};
+  V[2][2] = cpx<A>{-(t7.re + t8.re) * inv, -(t7.im + t8.im) * inv};
   if (neg) {
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -811,9 +869,10 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
   // Two-row links staged in LDS are rebuilt to full links while staging, once
   // per site and link instead of once per lane that reads them: the sweep
   // then sees the native layout (GLS).
-  constexpr int GLS = (LDS && GL == GL_NATIVE12) ? int(GL_NATIVE) : GL;
+  static_assert(GL != GL_NATIVE8 || LDS, "eight-real links are rebuilt only while staging in LDS");
+  constexpr int GLS = (LDS && (GL == GL_NATIVE12 || GL == GL_NATIVE8)) ? int(GL_NATIVE) : GL;
   constexpr int LC = link_cplx(GLS, N);  // complex numbers per link as the sweep reads them
-  if constexpr ((GL == GL_NATIVE || GL == GL_NATIVE12) && LDS) {
+  if constexpr (site_blocked(GL) && LDS) {
     static_assert(!SITE_MINOR, "LDS link staging needs site-major lanes");
     // Stage the 8 links of every site of this workgroup once; all k/R lanes of
     // a site then read them from LDS (broadcast) instead of from L1/L2.
@@ -832,8 +891,8 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
         dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
       }
     } else {
-      // One thread per (site, link): read the two stored rows, rebuild row 2
-      // (and the t-boundary sign), store the full link in the native slot.
+      // One thread per (site, link): read the stored reals, rebuild the full
+      // link (and the t-boundary sign), store it in the native slot.
       using W = typename std::conditional<std::is_same<CG, c64>::value, double, float>::type;
       CG *dst = reinterpret_cast<CG *>(smem);
       for (uint32_t i = threadIdx.x; i < nsite * 8; i += blockDim.x) {
@@ -846,7 +905,8 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
           neg = d == 3 ? int(t) == a.Lt - 1 : t == 0;
         }
         cpx<W> V[N][N];
-        load_link_native12<N, CG, W>(V, src, int(d), neg);  // d is the link's slot in the site block
+        if constexpr (GL == GL_NATIVE12) load_link_native12<N, CG, W>(V, src, int(d), neg);  // d: slot in the block
+        else load_link_native8<N, CG, W>(V, src, int(d), neg);
         CG *q = dst + (size_t(sloc) * 8 + d) * LC;
 #pragma unroll
         for (int r = 0; r < N; ++r)
@@ -871,7 +931,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
       g.xown = (lw < xstride || g.xh == 0) ? 1 : 0;
       g.xsrc = g.xown ? lw : lw - xstride;
     }
-    if constexpr ((GL == GL_NATIVE || GL == GL_NATIVE12) && !LDS)
+    if constexpr (site_blocked(GL) && !LDS)
       blk8 = static_cast<const CG *>(a.gauge) + (size_t(a.parity) * a.S + g.idx) * 8 * LC;
 
     const A s = A(a.sign);
@@ -883,7 +943,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};  // x + (-0) == x: folds
 
-    if constexpr (SL == SL_PACKED && (GL == GL_NATIVE || GL == GL_NATIVE12) && std::is_same<CS, CG>::value &&
+    if constexpr (SL == SL_PACKED && site_blocked(GL) && std::is_same<CS, CG>::value &&
                   (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) && std::is_same<A, float>::value) {
       using PV = typename Pair<CS>::V;
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
@@ -893,7 +953,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
     } else {
       cpx<A> psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
       load_spinor<N, R, SL, CS, A>(psi0, a, a.in, g.nbr[0], slot);
-      constexpr bool LL = LDS && (GL == GL_NATIVE || GL == GL_NATIVE12);
+      constexpr bool LL = LDS && site_blocked(GL);
       if constexpr (!LL && !(GL == GL_NATIVE && N >= kRowLinkN)) fetch_link<0, N, GL, CG, A>(V0, a, g, blk8);
       sweep<0, N, R, SL, GLS, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi0, V0, psi1, V1);
     }

@@ -24,6 +24,10 @@ int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, co
 // += the number of links further than tol from SU(3).
 int compress_gauge12(void *dst, int dst_prec, const void *native64, const int *L, int t_sign, double tol, int *bad,
                      hipStream_t st, std::string *err);
+// Same input -> eight-real layout [2][S][8][4] (U01, U02, U10, (arg U00, arg U20)); bad[0] += links further
+// than tol from SU(3), bad[1] += links with |U01|^2 + |U02|^2 < min_n (the rebuild divides by it).
+int compress_gauge8(void *dst, int dst_prec, const void *native64, const int *L, int t_sign, double tol,
+                    double min_n, int *bad, hipStream_t st, std::string *err);
 // n complex numbers, element-wise precision conversion.
 int convert_complex(void *dst, int dst_prec, const void *src, int src_prec, long long n, hipStream_t st,
                     std::string *err);

@@ -317,13 +317,11 @@ int pack_gauge(void *dst, int dst_prec, const void *src, int src_prec, int N, co
 // crosses the t boundary (slot 3 at t = Lt - 1, slot 7 at t = 0) is divided by
 // t_sign first. Every link must then be in SU(3) to within tol: rows 0 and 1
 // orthonormal and row 2 = conj(row 0 x row 1); each violation counts into *bad.
-template <class CD>
-__global__ void __launch_bounds__(256) gauge_compress12_kernel(CD *__restrict__ dst, const c64 *__restrict__ src,
-                                                               int subLx, int Ly, int Lz, int Lt, int t_sign,
-                                                               double tol, long long nlinks, int *bad)
+// Link i of a native FP64 SU(3) gauge [2][S][8][3][3], divided by t_sign when
+// it crosses the t boundary (slot 3 at t = Lt - 1, slot 7 at t = 0).
+__device__ __forceinline__ void load_su3_link(double (&re)[3][3], double (&im)[3][3], const c64 *src, long long i,
+                                              int subLx, int Ly, int Lz, int Lt, int t_sign)
 {
-  const long long i = blockIdx.x * 256LL + threadIdx.x;
-  if (i >= nlinks) return;
   const long long S = 1LL * subLx * Ly * Lz * Lt;
   const int slot = int(i % 8);
   const long long s = (i / 8) % S;
@@ -331,12 +329,16 @@ __global__ void __launch_bounds__(256) gauge_compress12_kernel(CD *__restrict__ 
   const bool crosses = (slot == 3 && t == Lt - 1) || (slot == 7 && t == 0);
   const double sg = (crosses && t_sign < 0) ? -1.0 : 1.0;
   const c64 *u = src + size_t(i) * 9;
-  double re[3][3], im[3][3];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) {
       re[r][c] = sg * u[r * 3 + c].x;
       im[r][c] = sg * u[r * 3 + c].y;
     }
+}
+
+// Distance from SU(3): rows 0 and 1 orthonormal, row 2 = conj(row 0 x row 1).
+__device__ __forceinline__ double su3_distance(const double (&re)[3][3], const double (&im)[3][3])
+{
   double err = 0.0;
   for (int a = 0; a < 2; ++a)
     for (int b = a; b < 2; ++b) {
@@ -353,7 +355,19 @@ __global__ void __launch_bounds__(256) gauge_compress12_kernel(CD *__restrict__ 
     const double ci = re[0][j1] * im[1][j2] + im[0][j1] * re[1][j2] - (re[0][j2] * im[1][j1] + im[0][j2] * re[1][j1]);
     err = fmax(err, fabs(cr - re[2][j]) + fabs(-ci - im[2][j]));
   }
-  if (!(err <= tol)) atomicAdd(bad, 1);
+  return err;
+}
+
+template <class CD>
+__global__ void __launch_bounds__(256) gauge_compress12_kernel(CD *__restrict__ dst, const c64 *__restrict__ src,
+                                                               int subLx, int Ly, int Lz, int Lt, int t_sign,
+                                                               double tol, long long nlinks, int *bad)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nlinks) return;
+  double re[3][3], im[3][3];
+  load_su3_link(re, im, src, i, subLx, Ly, Lz, Lt, t_sign);
+  if (!(su3_distance(re, im) <= tol)) atomicAdd(bad, 1);
   CD *d = dst + size_t(i) * 6;
   for (int r = 0; r < 2; ++r)
     for (int c = 0; c < 3; ++c) d[r * 3 + c] = StCvt<CD>::go(cpx<double>{re[r][c], im[r][c]});
@@ -379,6 +393,54 @@ int compress_gauge12(void *dst, int dst_prec, const void *native64, const int *L
       return 1;
   }
 #undef QM_C12
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    if (err) *err = std::string("gauge compression launch failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// Eight-real layout: U01, U02, U10 and (arg U00, arg U20) per link.
+template <class CD>
+__global__ void __launch_bounds__(256) gauge_compress8_kernel(CD *__restrict__ dst, const c64 *__restrict__ src,
+                                                              int subLx, int Ly, int Lz, int Lt, int t_sign,
+                                                              double tol, double min_n, long long nlinks, int *bad)
+{
+  const long long i = blockIdx.x * 256LL + threadIdx.x;
+  if (i >= nlinks) return;
+  double re[3][3], im[3][3];
+  load_su3_link(re, im, src, i, subLx, Ly, Lz, Lt, t_sign);
+  if (!(su3_distance(re, im) <= tol)) atomicAdd(bad, 1);
+  const double n = re[0][1] * re[0][1] + im[0][1] * im[0][1] + re[0][2] * re[0][2] + im[0][2] * im[0][2];
+  if (!(n >= min_n)) atomicAdd(bad + 1, 1);
+  CD *d = dst + size_t(i) * 4;
+  d[0] = StCvt<CD>::go(cpx<double>{re[0][1], im[0][1]});
+  d[1] = StCvt<CD>::go(cpx<double>{re[0][2], im[0][2]});
+  d[2] = StCvt<CD>::go(cpx<double>{re[1][0], im[1][0]});
+  d[3] = StCvt<CD>::go(cpx<double>{atan2(im[0][0], re[0][0]), atan2(im[2][0], re[2][0])});
+}
+
+int compress_gauge8(void *dst, int dst_prec, const void *native64, const int *L, int t_sign, double tol,
+                    double min_n, int *bad, hipStream_t st, std::string *err)
+{
+  const long long S = 1LL * L[0] / 2 * L[1] * L[2] * L[3];
+  const long long nlinks = 2LL * S * 8;
+  const unsigned nblk = unsigned((nlinks + 255) / 256);
+  const c64 *src = static_cast<const c64 *>(native64);
+#define QM_C8(CD)                                                                                               \
+  hipLaunchKernelGGL((gauge_compress8_kernel<CD>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst), src, \
+                     L[0] / 2, L[1], L[2], L[3], t_sign, tol, min_n, nlinks, bad)
+  switch (dst_prec) {
+    case PREC_DOUBLE: QM_C8(c64); break;
+    case PREC_SINGLE: QM_C8(c32); break;
+    case PREC_HALF: QM_C8(c16); break;
+    case PREC_BF16: QM_C8(cb16); break;
+    default:
+      if (err) *err = "unsupported gauge precision";
+      return 1;
+  }
+#undef QM_C8
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     if (err) *err = std::string("gauge compression launch failed: ") + hipGetErrorString(e);

@@ -94,6 +94,29 @@ def test_gauge12_sizes_and_validation(lib):
     assert lib.qcuMrhsDslash12(16, 16, 16, _lib.SINGLE, 2, 0, 0, 2, g3, _lib.tile(), None) != 0
 
 
+def test_gauge8_sizes_and_validation(lib):
+    """Eight-real links: 4 complex slots per link, SU(3) only, rebuilt while
+    staging (tiles without LDS staging are refused); bad arguments are
+    rejected before anything reaches the GPU."""
+    from mrhs_qcu_amd import _lib
+
+    g3 = _lib.geom((8, 4, 4, 8), 3)
+    S = 4 * 4 * 4 * 8
+    assert lib.qcuMrhsGauge8Bytes(g3, _lib.SINGLE) == S * 2 * 8 * 4 * 8
+    assert lib.qcuMrhsGauge8Bytes(g3, _lib.DOUBLE) * 9 == lib.qcuMrhsGaugeBytes(g3, _lib.DOUBLE) * 4
+    g2 = _lib.geom((8, 4, 4, 8), 2)
+    assert lib.qcuMrhsGauge8Bytes(g2, _lib.SINGLE) == 0
+    assert lib.qcuMrhsPackGauge8(16, _lib.SINGLE, 16, _lib.DOUBLE, g2, -1, None) != 0
+    assert "nColors = 3" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsPackGauge8(16, _lib.SINGLE, 16, _lib.DOUBLE, g3, 0, None) != 0
+    assert "t_sign" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash8(16, 16, 16, _lib.SINGLE, 2, 0, 0, 1, g2, _lib.tile(1, 128, 1), None) != 0
+    assert "nColors = 3" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash8(16, 16, 16, _lib.SINGLE, 2, 0, 0, 1, g3, _lib.tile(1, 128, 0), None) != 0
+    assert "link_lds" in lib.qcuMrhsLastError().decode()
+    assert lib.qcuMrhsDslash8(16, 16, 16, _lib.SINGLE, 2, 0, 0, 2, g3, _lib.tile(1, 128, 1), None) != 0
+
+
 def test_clover_sizes_and_validation(lib):
     from mrhs_qcu_amd import _lib
 
