@@ -45,7 +45,10 @@ __host__ __device__ constexpr int link_cplx(int GL, int N)
   return GL == GL_NATIVE12 ? 2 * N : (GL == GL_NATIVE8 ? 4 : N * N);
 }
 // Layouts that store each site's 8 links as one block ([2][S][8][...]).
-__host__ __device__ constexpr bool site_blocked(int GL) { return GL == GL_NATIVE || GL == GL_NATIVE12 || GL == GL_NATIVE8; }
+__host__ __device__ constexpr bool site_blocked(int GL)
+{
+  return GL == GL_NATIVE || GL == GL_NATIVE12 || GL == GL_NATIVE8;
+}
 
 constexpr int kMaxPtrs = 32;  // RHS per launch for the pointer layouts
 struct PtrPack {
