@@ -476,6 +476,9 @@ def test_xcd_cyclic_order(lib, dev):
     for R, block, lds, tb, zb in ((2, 64, 1, 2, 3), (2, 64, 0, 2, 3), (1, 128, 1, 2, 2), (2, 256, 1, 2, 3)):
         got = run_packed(lib, dev, g, psi, L, N, 0, 1, _lib.SINGLE, tile=_lib.tile(R, block, lds, tb, zb, 8, 0, 1))
         assert rel_err(got, want) < TOL[1], (R, block, lds, tb, zb)
+    for prec, R in ((_lib.DOUBLE, 1), (_lib.DOUBLE, 2), (_lib.HALF, 2), (_lib.BF16, 2)):
+        got = run_packed(lib, dev, g, psi, L, N, 0, 1, prec, tile=_lib.tile(R, 64, 1, 2, 3, 8, 0, 1))
+        assert rel_err(got, want) < TOL[prec], (prec, R)
 
     L, k = (24, 24, 24, 48), 16
     geom = _lib.geom(L, N)
