@@ -91,6 +91,8 @@ struct DslashArgs {
                 // time folded into the links), which two stored rows cannot express
   int aos_vec;  // pointer layouts: every buffer 16-byte aligned (16-byte loads and stores)
   int plain_out;  // packed output is D(in) as it is: no xpay operand, scales, norms or fused update
+  uint32_t efac_off;  // > 0 (LDS kernels): byte offset in dynamic LDS of the staged per-RHS epilogue
+                      // factors [4][k] in the accumulation type (xb sc_in, xa sc_x, 1 / sc_out, rx_alpha)
   uint32_t xcd_gran;  // > 0: XCD-cyclic workgroup order in chunks of xcd_gran workgroups (xcd_cyclic)
 };
 
@@ -917,6 +919,17 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
           for (int c = 0; c < N; ++c) q[r * N + c] = StCvt<CG>::go(V[r][c]);
       }
     }
+    if (a.efac_off) {
+      // The per-RHS factors of the epilogue, loaded here beside the links so
+      // that their latency is not paid after the last neighbour load.
+      A *ef = reinterpret_cast<A *>(smem + a.efac_off);
+      for (int i = threadIdx.x; i < a.k; i += blockDim.x) {
+        ef[i] = A(a.xb) * (a.sc_in ? A(a.sc_in[i]) : A(1));
+        ef[a.k + i] = A(a.xa) * (a.sc_x ? A(a.sc_x[i]) : A(1));
+        ef[2 * a.k + i] = a.sc_out ? A(1) / A(a.sc_out[i]) : A(1);
+        ef[3 * a.k + i] = a.rx_alpha ? A(a.rx_alpha[i]) : A(0);
+      }
+    }
     __syncthreads();
     if (active) blk8 = reinterpret_cast<const CG *>(smem) + size_t(sl - first) * 8 * LC;
   }
@@ -963,12 +976,21 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
 
     // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.
     A fin[R], fx[R], fo[R];
+    const A *ef = nullptr;
+    if constexpr (site_blocked(GL) && LDS)
+      if (a.efac_off) ef = reinterpret_cast<const A *>(smem + a.efac_off);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int rhs = int(slot) * R + r;
-      fin[r] = A(a.xb) * (a.sc_in ? A(a.sc_in[rhs]) : A(1));
-      fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
-      fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
+      if (ef) {
+        fin[r] = ef[rhs];
+        fx[r] = ef[a.k + rhs];
+        fo[r] = ef[2 * a.k + rhs];
+      } else {
+        fin[r] = A(a.xb) * (a.sc_in ? A(a.sc_in[rhs]) : A(1));
+        fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
+        fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
+      }
     }
     bool fused = false;
     if constexpr (SL == SL_PACKED && ((std::is_same<CO, c32>::value && std::is_same<A, float>::value) ||
@@ -982,7 +1004,7 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
             a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
         A al[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) al[r] = A(a.rx_alpha[int(slot) * R + r]);
+        for (int r = 0; r < R; ++r) al[r] = ef ? ef[3 * a.k + int(slot) * R + r] : A(a.rx_alpha[int(slot) * R + r]);
 #pragma unroll
         for (int sc = 0; sc < 4 * N; ++sc) {
           CVec<CS, R> xv;
