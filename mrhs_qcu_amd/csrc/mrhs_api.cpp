@@ -519,27 +519,52 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   if (cands.empty()) return api_fail("no instantiated tile for this shape");
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return api_fail("hipEventCreate failed");
-  double best_t = 1e30;
-  Tile best_tile = cands[0];
-  const int reps = 5;
-  for (const Tile &t : cands) {
-    // Warm-up launch also validates the tile.
-    if (run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, 0, 0, g, &t, st, &err)) continue;
+  // Time `reps` launches (alternating parity) of a tile; -1: the tile is not
+  // valid here, -2: a launch failed on the device.
+  auto time_tile = [&](const Tile &t, int reps) -> double {
+    // The warm-up launch also validates the tile.
+    if (run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, 0, 0, g, &t, st, &err)) return -1.0;
     (void)hipEventRecord(e0, st);
     for (int r = 0; r < reps; ++r)
       run_packed_dslash(out, in, nullptr, 0.0, 1.0, gauge, prec, nrhs, r & 1, 0, g, &t, st, &err);
     (void)hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess) {
+    if (hipEventSynchronize(e1) != hipSuccess) return -2.0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return 1e3 * ms / reps;
+  };
+  // Two rounds: every candidate briefly, then the few fastest again with
+  // more launches, so that near-ties are decided on a steadier measurement.
+  std::vector<std::pair<double, Tile>> timed;
+  for (const Tile &t : cands) {
+    const double us = time_tile(t, 5);
+    if (us == -2.0) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       return api_fail("tuning launch failed");
     }
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    const double us = 1e3 * ms / reps;
-    if (us < best_t) {
+    if (us >= 0.0) timed.emplace_back(us, t);
+  }
+  if (timed.empty()) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return api_fail("no tile launched for this shape: " + err);
+  }
+  std::stable_sort(timed.begin(), timed.end(),
+                   [](const std::pair<double, Tile> &x, const std::pair<double, Tile> &y) { return x.first < y.first; });
+  double best_t = 1e30;
+  Tile best_tile = timed[0].second;
+  const size_t finalists = timed.size() < 6 ? timed.size() : 6;
+  for (size_t i = 0; i < finalists; ++i) {
+    const double us = time_tile(timed[i].second, 20);
+    if (us == -2.0) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return api_fail("tuning launch failed");
+    }
+    if (us >= 0.0 && us < best_t) {
       best_t = us;
-      best_tile = t;
+      best_tile = timed[i].second;
     }
   }
   (void)hipEventDestroy(e0);
