@@ -60,6 +60,8 @@ def parse():
                          "(skips the tuner)")
     ap.add_argument("--faces-only", action="store_true",
                     help="only the split-lattice face exchange and clover timings")
+    ap.add_argument("--stream", type=float, nargs="*", default=None,
+                    help="only the stream-bandwidth ceiling over buffers of these sizes in MiB")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
     ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
@@ -89,29 +91,33 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
-    a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
-    b = torch.empty_like(a)
+    # Two fixed Gaussian inputs and two outputs: step i applies the operator of
+    # parity i & 1 to ins[i & 1], so every timed launch sees the same finite
+    # data (feeding outputs back in would grow the field ~4x per application).
+    ins = [torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt) for _ in range(2)]
+    outs = [torch.empty_like(ins[0]) for _ in range(2)]
     if tile is None:
         best = _lib.Tile()
         us = ctypes.c_double()
-        _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, ctypes.byref(best),
-                                   ctypes.byref(us), sp))
+        _lib.check(lib.qcuMrhsTune(outs[0].data_ptr(), ins[0].data_ptr(), gn.data_ptr(), prec, k, geom,
+                                   ctypes.byref(best), ctypes.byref(us), sp))
         tp = None
     else:
         best = _lib.tile(*tile)
         tp = ctypes.byref(best)
-    bufs = [a, b]
+    args = [(outs[p].data_ptr(), ins[p].data_ptr(), gn.data_ptr(), prec, k, p, 0, geom, tp, sp) for p in (0, 1)]
     for i in range(warmup):
-        _lib.check(lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k,
-                                     i & 1, 0, geom, tp, sp))
+        _lib.check(lib.qcuMrhsDslash(*args[i & 1]))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rcs = [0] * steps
     e0.record(st)
     for i in range(steps):
-        lib.qcuMrhsDslash(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), gn.data_ptr(), prec, k, i & 1, 0,
-                          geom, tp, sp)
+        rcs[i] = lib.qcuMrhsDslash(*args[i & 1])
     e1.record(st)
     torch.cuda.synchronize()
+    for rc in rcs:
+        _lib.check(rc)
     ms = e0.elapsed_time(e1) / steps
     return ms, best.as_tuple()
 
@@ -137,19 +143,21 @@ def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0, rea
     g12 = torch.empty(nbytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
     # The generator folds antiperiodic time into the links: t_sign = -1.
     _lib.check(pack(g12.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, -1, sp))
-    a = torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt)
-    bufs = [a, torch.empty_like(a)]
+    ins = [torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt) for _ in range(2)]
+    outs = [torch.empty_like(ins[0]) for _ in range(2)]
+    args = [(outs[p].data_ptr(), ins[p].data_ptr(), g12.data_ptr(), prec, k, p, 0, -1, geom, None, sp) for p in (0, 1)]
     for i in range(warmup):
-        _lib.check(run(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0, -1,
-                       geom, None, sp))
+        _lib.check(run(*args[i & 1]))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rcs = [0] * steps
     e0.record(st)
     for i in range(steps):
-        run(bufs[(i + 1) & 1].data_ptr(), bufs[i & 1].data_ptr(), g12.data_ptr(), prec, k, i & 1, 0, -1, geom, None,
-            sp)
+        rcs[i] = run(*args[i & 1])
     e1.record(st)
     torch.cuda.synchronize()
+    for rc in rcs:
+        _lib.check(rc)
     return e0.elapsed_time(e1) / steps
 
 
@@ -284,14 +292,17 @@ def faces_benchmark(lib, _lib, torch, L=(24, 24, 24, 24), grid=(1, 1, 1, 2), N=3
 
     def timed(fn):
         for _ in range(warmup):
-            fn()
+            _lib.check(fn())
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        rcs = [0] * steps
         e0.record(st)
-        for _ in range(steps):
-            fn()
+        for i in range(steps):
+            rcs[i] = fn()
         e1.record(st)
         torch.cuda.synchronize()
+        for rc in rcs:
+            _lib.check(rc)
         return e0.elapsed_time(e1) / steps
 
     ms_int = timed(lambda: lib.qcuMrhsDslash(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, 0, 0, geom, None, sp))
@@ -326,11 +337,14 @@ def clover_benchmark(lib, _lib, torch, L=(24, 24, 24, 48), N=3, k=16, steps=200,
         _lib.check(lib.qcuMrhsCloverApply(b.data_ptr(), a.data_ptr(), cl.data_ptr(), _lib.SINGLE, k, i & 1, geom, sp))
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rcs = [0] * steps
     e0.record(st)
     for i in range(steps):
-        lib.qcuMrhsCloverApply(b.data_ptr(), a.data_ptr(), cl.data_ptr(), _lib.SINGLE, k, i & 1, geom, sp)
+        rcs[i] = lib.qcuMrhsCloverApply(b.data_ptr(), a.data_ptr(), cl.data_ptr(), _lib.SINGLE, k, i & 1, geom, sp)
     e1.record(st)
     torch.cuda.synchronize()
+    for rc in rcs:
+        _lib.check(rc)
     ms = e0.elapsed_time(e1) / steps
     nbytes = S * (k * 4 * N * 8 * 2 + 8 * N * N * 8)
     return {"workload": f"su{N}_clover_apply_{'x'.join(map(str, L))}_nrhs{k}_fp32", "ms_per_step": ms,
@@ -371,18 +385,22 @@ def batch_benchmark(lib, _lib, torch, steps=50, warmup=5):
             _lib.check(lib.qcuMrhsBatchRun(h, sp))
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        rcs = []
         e0.record(st)
         for _ in range(steps):
-            lib.qcuMrhsBatchRun(h, sp)
+            rcs.append(lib.qcuMrhsBatchRun(h, sp))
         e1.record(st)
         torch.cuda.synchronize()
         ms_graph = e0.elapsed_time(e1) / steps
         e0.record(st)
         for _ in range(steps):
             for p in probs:
-                lib.qcuMrhsDslash(p.out, p.in_, p.gauge, _lib.SINGLE, p.nrhs, p.parity, p.dagger, p.geom, None, sp)
+                rcs.append(lib.qcuMrhsDslash(p.out, p.in_, p.gauge, _lib.SINGLE, p.nrhs, p.parity, p.dagger, p.geom,
+                                             None, sp))
         e1.record(st)
         torch.cuda.synchronize()
+        for rc in rcs:
+            _lib.check(rc)
         ms_seq = e0.elapsed_time(e1) / steps
     finally:
         lib.qcuMrhsBatchDestroy(h)
@@ -426,6 +444,43 @@ def qcu_h_end_to_end(lib, _lib, torch, L=(24, 24, 24, 48), N=3, k=16, reps=20):
             "ms_per_call": wall * 1e3, "sites_rhs_per_s": S * k / wall}
 
 
+def cpu_model():
+    """Host CPU model name from /proc/cpuinfo (or 'unknown')."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+STREAM_MODES = ("read", "write", "copy", "read_write")
+
+
+def stream_benchmark(lib, _lib, torch, sizes_mb=(510,), reps=20):
+    """Streaming ceiling of this GPU: the library's 16-byte-access stream
+    kernels (read, write, copy, in-place read+write) over buffers of the given
+    sizes; GB/s counts bytes read plus bytes written."""
+    dev = torch.device("cuda:0")
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for mb in sizes_mb:
+        nbytes = int(mb * 2**20) // 16 * 16
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        b = torch.zeros_like(a)
+        row = {}
+        for mode, name in enumerate(STREAM_MODES):
+            gbs = ctypes.c_double()
+            _lib.check(lib.qcuMrhsStreamBandwidth(mode, b.data_ptr(), a.data_ptr(), nbytes, reps, ctypes.byref(gbs),
+                                                  sp))
+            row[name] = round(gbs.value, 1)
+        out[f"{mb}MiB"] = row
+        del a, b
+    return out
+
+
 def cpu_baseline(L, N, k, budget_s=12.0):
     """FP32 C oracle (OpenMP) on the host: same workload, bounded repetitions."""
     import numpy as np
@@ -450,7 +505,7 @@ def cpu_baseline(L, N, k, budget_s=12.0):
             break
     t = float(np.median(times))
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
-    return {"value": S * k / t, "unit": "sites*RHS/s", "cores": cores, "kind": "port",
+    return {"value": S * k / t, "unit": "sites*RHS/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"FP32 C restatement of the reference applyWilson (oracle/oracle.c), OpenMP, "
                       f"{'x'.join(map(str, L))} SU({N}) nrhs={k}, median of {len(times)} applications "
                       f"({sum(times):.1f} s)"}
@@ -494,6 +549,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+        return
+    if args.stream is not None:
+        sizes = tuple(args.stream) or (16, 64, 128, 510, 2048)
+        print(json.dumps({"stream_GBs": stream_benchmark(lib, _lib, torch, sizes_mb=sizes)}), flush=True)
         return
     if args.faces_only:
         print(json.dumps(faces_benchmark(lib, _lib, torch)), flush=True)
@@ -560,6 +619,7 @@ def main():
             "ms_per_step": ms3, "sites_rhs_per_s": S2 * kc2 / (ms3 * 1e-3),
             "hbm_frac_cache_resident": bytes_alg(S2, kc2, N, 4, 4, 4) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tile": t3}
+        extra["stream_ceiling_GBs"] = stream_benchmark(lib, _lib, torch, sizes_mb=(510, 2048))
         extra["batch"] = batch_benchmark(lib, _lib, torch)
         extra["faces"] = faces_benchmark(lib, _lib, torch)
         extra["clover"] = clover_benchmark(lib, _lib, torch)

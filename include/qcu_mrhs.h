@@ -195,13 +195,22 @@ int qcuMrhsDslashFaces(void *out, const void *const *from_bwd, const void *const
 /* Times every instantiated tile on the given fields (out is overwritten),
  * records the fastest for (N, nrhs, prec, lattice) and returns it. With
  * QCU_MRHS_TUNE_CACHE=<file> in the environment, results persist in that text
- * file (keyed by GPU architecture) and are reused by later processes. */
+ * file (keyed by GPU architecture) and are reused by later processes. With
+ * QCU_MRHS_TUNE_LOG=1 every candidate tile and its time go to stderr. */
 int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *g,
                 qcuMrhsTile *best, double *best_us, void *stream);
 /* Tile the dispatcher uses for a shape (tuned if known, else the default). */
 int qcuMrhsGetTile(const qcuMrhsGeom *g, int prec, int nrhs, qcuMrhsTile *tile);
 /* Number of instantiated dslash kernels (for diagnostics). */
 int qcuMrhsKernelCount(void);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Streaming bandwidth of the GPU, the ceiling the dslash's roofline fraction
+ * is read against: mode 0 reads `bytes` of src, 1 writes `bytes` of dst,
+ * 2 copies src to dst, 3 reads and rewrites dst in place; 16-byte accesses
+ * over every CU, one warm-up pass, then `reps` timed passes. *gbs receives the
+ * bytes moved (read + written) per second in GB/s. Synchronous. */
+int qcuMrhsStreamBandwidth(int mode, void *dst, const void *src, size_t bytes, int reps, double *gbs, void *stream);
 
 /* ---- solver ------------------------------------------------------------- */
 typedef struct qcuMrhsCgParam_s {

@@ -58,7 +58,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
                       "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
                       "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
-                      "qcuMrhsCgSolveClover"]
+                      "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth"]
 
 _lib = None
 
@@ -116,6 +116,7 @@ def _declare(lib):
         "qcuMrhsCloverCompute": (I, [P, P, I, P, I, D, G, P]),
         "qcuMrhsCloverApply": (I, [P, P, P, I, I, I, G, P]),
         "qcuMrhsCgSolveClover": (I, [P, P, P, P, P, I, G, ctypes.POINTER(CgParam), ctypes.POINTER(D), P]),
+        "qcuMrhsStreamBandwidth": (I, [I, P, P, ctypes.c_size_t, I, ctypes.POINTER(D), P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])

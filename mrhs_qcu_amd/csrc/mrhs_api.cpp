@@ -536,8 +536,16 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   // Two rounds: every candidate briefly, then the few fastest again with
   // more launches, so that near-ties are decided on a steadier measurement.
   std::vector<std::pair<double, Tile>> timed;
+  const char *logenv = getenv("QCU_MRHS_TUNE_LOG");
+  const bool log = logenv && logenv[0] && logenv[0] != '0';
+  auto log_tile = [&](const char *round, const Tile &t, double us) {
+    if (log)
+      fprintf(stderr, "qcuMrhsTune %s N=%d k=%d prec=%d tile=%d,%d,%d,%d,%d,%d,%d,%d us=%.2f\n", round, g.N, nrhs,
+              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream, us);
+  };
   for (const Tile &t : cands) {
     const double us = time_tile(t, 5);
+    log_tile("first", t, us);
     if (us == -2.0) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -557,6 +565,7 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   const size_t finalists = timed.size() < 6 ? timed.size() : 6;
   for (size_t i = 0; i < finalists; ++i) {
     const double us = time_tile(timed[i].second, 20);
+    log_tile("final", timed[i].second, us);
     if (us == -2.0) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -1,0 +1,107 @@
+// Streaming-bandwidth kernels: the ceiling against which the dslash's
+// roofline fraction is read. Every access is a 16-byte vector load or store;
+// each thread handles kUnroll chunks spaced one workgroup apart (consecutive
+// lanes on consecutive 16-byte chunks), and the grid covers the buffer once,
+// so every CU takes part and no chunk is touched twice per pass.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "api_internal.h"
+
+namespace qmrhs {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int kBlock = 256;
+constexpr int kUnroll = 4;
+
+// mode 0: read src (the sum of what was read is stored only if it equals a
+//         value no finite input produces, so the loads cannot be dropped);
+// mode 1: write dst; mode 2: copy src -> dst; mode 3: dst = dst * one in place.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) stream_kernel(f4 *dst, const f4 *src, size_t n16, float one)
+{
+  const size_t base = size_t(blockIdx.x) * kBlock * kUnroll + threadIdx.x;
+  f4 v[kUnroll];
+  if constexpr (MODE == 0 || MODE == 2 || MODE == 3) {
+    const f4 *p = MODE == 3 ? dst : src;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = base + size_t(u) * kBlock;
+      v[u] = i < n16 ? p[i] : f4{0.f, 0.f, 0.f, 0.f};
+    }
+  }
+  if constexpr (MODE == 0) {
+    f4 s = v[0];
+#pragma unroll
+    for (int u = 1; u < kUnroll; ++u) s += v[u];
+    const float t = s.x + s.y + s.z + s.w;
+    if (t == 3.0e38f * one) dst[0] = s;
+  } else {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = base + size_t(u) * kBlock;
+      if (i >= n16) break;
+      if constexpr (MODE == 1) dst[i] = f4{one, one, one, one};
+      else if constexpr (MODE == 2) dst[i] = v[u];
+      else dst[i] = v[u] * one;
+    }
+  }
+}
+
+int launch_stream(int mode, void *dst, const void *src, size_t n16, hipStream_t st)
+{
+  const size_t per_block = size_t(kBlock) * kUnroll;
+  const size_t grid = (n16 + per_block - 1) / per_block;
+  f4 *d = static_cast<f4 *>(dst);
+  const f4 *s = static_cast<const f4 *>(src);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(stream_kernel<0>, dim3(unsigned(grid)), dim3(kBlock), 0, st, d, s, n16, 1.0f); break;
+    case 1: hipLaunchKernelGGL(stream_kernel<1>, dim3(unsigned(grid)), dim3(kBlock), 0, st, d, s, n16, 1.0f); break;
+    case 2: hipLaunchKernelGGL(stream_kernel<2>, dim3(unsigned(grid)), dim3(kBlock), 0, st, d, s, n16, 1.0f); break;
+    default: hipLaunchKernelGGL(stream_kernel<3>, dim3(unsigned(grid)), dim3(kBlock), 0, st, d, s, n16, 1.0f); break;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace
+}  // namespace qmrhs
+
+using namespace qmrhs;
+
+extern "C" int qcuMrhsStreamBandwidth(int mode, void *dst, const void *src, size_t bytes, int reps, double *gbs,
+                                      void *stream)
+{
+  if (mode < 0 || mode > 3) return api_fail("stream mode must be 0 (read), 1 (write), 2 (copy) or 3 (read+write)");
+  if (bytes < 16 || bytes % 16) return api_fail("stream bytes must be a positive multiple of 16");
+  if (bytes / 16 / (size_t(kBlock) * kUnroll) >= (size_t(1) << 31)) return api_fail("stream buffer too large");
+  if (reps < 1) return api_fail("reps < 1");
+  const bool need_src = mode == 0 || mode == 2, need_dst = mode != 0;
+  if ((need_src && !src) || (need_dst && !dst) || (mode == 0 && !dst))
+    return api_fail("null stream buffer (mode 0 needs 16 bytes at dst for its guard store)");
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16)
+    return api_fail("stream buffers must be 16-byte aligned");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t n16 = bytes / 16;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return api_fail("hipEventCreate failed");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return api_fail("hipEventCreate failed");
+  }
+  int rc = launch_stream(mode, dst, src, n16, st);  // warm-up pass
+  if (!rc) rc = hipEventRecord(e0, st) != hipSuccess;
+  for (int r = 0; r < reps && !rc; ++r) rc = launch_stream(mode, dst, src, n16, st);
+  if (!rc) rc = hipEventRecord(e1, st) != hipSuccess;
+  if (!rc) rc = hipEventSynchronize(e1) != hipSuccess;
+  float ms = 0.f;
+  if (!rc) rc = hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return api_fail("stream kernel launch or timing failed");
+  const double moved = double(bytes) * (mode == 2 || mode == 3 ? 2.0 : 1.0) * reps;
+  if (gbs) *gbs = moved / (double(ms) * 1e-3) / 1e9;
+  return 0;
+}
