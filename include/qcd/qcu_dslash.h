@@ -71,4 +71,12 @@ void instantiateDslash(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &p
                        double kappa, void *mrhsFermionIn, void *mrhsFermionOut, void *gauge, DSLASH_TYPE dtype,
                        QCU_PRECISION floatPrecision, int nColors, int nInputs, ExecutionStreams &streams);
 
+// Not in the reference: out_r = a * x_r + b * Dslash(in_r) for the same
+// pointer packs and layouts as instantiateDslash (mrhsFermionX: host array of
+// nInputs device pointers, one per RHS, in the input's layout), one launch.
+void instantiateDslashXpay(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &procDesc, int parity,
+                           int daggerFlag, double a, double b, void *mrhsFermionIn, void *mrhsFermionX,
+                           void *mrhsFermionOut, void *gauge, DSLASH_TYPE dtype, QCU_PRECISION floatPrecision,
+                           int nColors, int nInputs, ExecutionStreams &streams);
+
 }  // namespace qcu

@@ -38,6 +38,14 @@ class Dirac {
   virtual ~Dirac() {}
   virtual void Dslash(void *outputMRHS, void *inputMRHS, int parity, int daggerFlag, int nInputs) = 0;
   virtual void DslashXpay() = 0;
+  // Not in the reference (whose DslashXpay takes no operands): out_r = a * x_r
+  // + b * Dslash(in_r), same pointer packs as Dslash, xMRHS one pointer per RHS.
+  virtual void DslashXpay(void *outputMRHS, void *inputMRHS, void *xMRHS, double a, double b, int parity,
+                          int daggerFlag, int nInputs)
+  {
+    (void)outputMRHS, (void)inputMRHS, (void)xMRHS, (void)a, (void)b, (void)parity, (void)daggerFlag, (void)nInputs;
+    errorQcu("DslashXpay with operands is not implemented for this operator.\n");
+  }
   double kappa() const { return kappa_; }
 };
 
@@ -52,9 +60,12 @@ class DiracWilson : public Dirac {
   // outputMRHS / inputMRHS: host arrays of nInputs device pointers, each a
   // single-parity spinor in the reference SoA layout [4*N][V/2].
   virtual void Dslash(void *outputMRHS, void *inputMRHS, int parity, int daggerFlag, int nInputs);
-  // Empty in the reference (qcu_dirac.h:49); the fused x + a*D y lives in the
-  // extended C ABI (qcuMrhsDslashXpay) where its operands can be named.
+  // Empty in the reference (qcu_dirac.h:49), and kept empty: it names no operands.
   virtual void DslashXpay() {}
+  // out_r = a * x_r + b * Dslash(in_r) in one launch (reference SoA layout,
+  // like Dslash).
+  virtual void DslashXpay(void *outputMRHS, void *inputMRHS, void *xMRHS, double a, double b, int parity,
+                          int daggerFlag, int nInputs);
 };
 
 }  // namespace qcu

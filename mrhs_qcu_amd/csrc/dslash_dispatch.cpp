@@ -217,7 +217,12 @@ int launch_dslash(const DslashCall &c, std::string *err)
       return fail(err, "eight-real links are rebuilt while staging: the tile needs link_lds = 1"), 1;
     const int lds = c.tile.lds ? 1 : 0;
     KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
-    LaunchFn fn = find_kernel(key);
+    // The same kernel compiled for this exact RHS count, when one exists
+    // (bit-identical results, constant row offsets).
+    KernelKey key_kf = key;
+    key_kf.kf = c.k;
+    LaunchFn fn = find_kernel(key_kf);
+    if (!fn) fn = find_kernel(key);
     if (!fn) return fail(err, "no kernel instantiated for this precision / N / tile"), 1;
     a.k = c.k;
     a.P = c.k / R;
@@ -277,7 +282,12 @@ int launch_dslash(const DslashCall &c, std::string *err)
       if (!c.out_ptrs[r0 + r] || !c.in_ptrs[r0 + r]) return fail(err, "null spinor pointer"), 1;
       b.out_ptrs.p[r] = c.out_ptrs[r0 + r];
       b.in_ptrs.p[r] = const_cast<void *>(c.in_ptrs[r0 + r]);
+      if (c.x_ptrs) {
+        if (!c.x_ptrs[r0 + r]) return fail(err, "null xpay spinor pointer"), 1;
+        b.x_ptrs.p[r] = const_cast<void *>(c.x_ptrs[r0 + r]);
+      }
     }
+    b.xptr = c.x_ptrs ? 1 : 0;
     b.k = kk;
     b.P = kk;
     b.div_P = make_fastdiv(uint32_t(kk));

@@ -13,10 +13,11 @@ namespace qmrhs {
 
 struct KernelKey {
   int sl, gl, n, r, lds, site_minor, prec_s, prec_g, prec_o;
+  int kf = 0;  // > 0: instantiated for exactly this RHS count
   bool operator==(const KernelKey &o) const
   {
     return sl == o.sl && gl == o.gl && n == o.n && r == o.r && lds == o.lds && site_minor == o.site_minor &&
-           prec_s == o.prec_s && prec_g == o.prec_g && prec_o == o.prec_o;
+           prec_s == o.prec_s && prec_g == o.prec_g && prec_o == o.prec_o && kf == o.kf;
   }
 };
 
@@ -28,25 +29,27 @@ LaunchFn find_kernel(const KernelKey &key);
 AttrFn find_kernel_attr(const KernelKey &key);
 int registry_size();
 
-template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A, int KF>
 hipError_t launch_impl(const DslashArgs &a, dim3 g, dim3 b, size_t shm, hipStream_t st)
 {
-  hipLaunchKernelGGL((dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A>), g, b, shm, st, a);
+  hipLaunchKernelGGL((dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A, KF>), g, b, shm, st, a);
   return hipGetLastError();
 }
-template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A, int KF>
 hipError_t attr_impl(hipFuncAttributes *at)
 {
-  return hipFuncGetAttributes(at, reinterpret_cast<const void *>(&dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A>));
+  return hipFuncGetAttributes(at,
+                              reinterpret_cast<const void *>(&dslash_kernel<N, R, SL, GL, LDS, SM, CS, CG, CO, A, KF>));
 }
 
-template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A>
+template <int N, int R, int SL, int GL, bool LDS, bool SM, class CS, class CG, class CO, class A, int KF = 0>
 struct Reg {
   Reg()
   {
-    register_kernel(KernelKey{SL, GL, N, R, LDS ? 1 : 0, SM ? 1 : 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec,
-                              StoreInfo<CO>::prec},
-                    &launch_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A>, &attr_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A>);
+    KernelKey key{SL, GL, N, R, LDS ? 1 : 0, SM ? 1 : 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec, StoreInfo<CO>::prec};
+    key.kf = KF;
+    register_kernel(key, &launch_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A, KF>,
+                    &attr_impl<N, R, SL, GL, LDS, SM, CS, CG, CO, A, KF>);
   }
 };
 
@@ -87,6 +90,7 @@ struct DslashCall {
   const void *in = nullptr, *gauge = nullptr, *x = nullptr;
   void *const *out_ptrs = nullptr;       // host arrays of k device pointers (pointer layouts)
   const void *const *in_ptrs = nullptr;
+  const void *const *x_ptrs = nullptr;   // pointer layouts: per-RHS xpay operand (out = xa x + xb D in)
   const float *sc_in = nullptr, *sc_x = nullptr, *sc_out = nullptr;
   double xa = 0.0, xb = 1.0;
   double *partial = nullptr;  // [nblocks][k]; nblocks from dslash_num_blocks()
