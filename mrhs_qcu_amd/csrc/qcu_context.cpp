@@ -59,6 +59,11 @@ class Qcu {
   Dirac *dirac_;
   int nColors_;
   int diracPrec_;  // storage precision of the current Dirac operator (-1: none)
+  // Arguments of the last getDslash, to rebuild the operator when the gauge
+  // copies it points to are replaced by a new loadQcuGauge.
+  DSLASH_TYPE diracType_ = DSLASH_WILSON;
+  double diracMass_ = 0.0;
+  QCU_PRECISION diracFloatPrec_ = QCU_DOUBLE_PRECISION;
   std::unique_ptr<ExecutionStreams> streams_;
 
   void *gaugeSoA64_ = nullptr;              // reference SoA copy, FP64 (the reference's localGauge_)
@@ -131,7 +136,21 @@ class Qcu {
     gaugeNativeByPrec_.clear();
     gaugeNativeByPrec_[qmrhs::PREC_DOUBLE] = keep;
     CHECK_HIP(hipStreamSynchronize(st));
+    // An operator made by an earlier getDslash points to a converted copy
+    // freed above: rebuild it on the new field.
+    if (dirac_) makeDirac();
   }
+
+  void makeDirac()
+  {
+    delete dirac_;
+    dirac_ = nullptr;
+    dirac_ = new DiracWilson(soaGauge(diracPrec_), diracMass_, nColors_, diracFloatPrec_, lattDesc_, procDesc_,
+                             diracType_, streams());
+  }
+
+  // The current operator (null before getDslash).
+  Dirac *dirac() { return dirac_; }
 
   void *nativeBuffer(int prec)
   {
@@ -180,10 +199,11 @@ class Qcu {
     if (prec < 0) errorQcu("Now we only support half, single and double precision.\n");
     if (!stagingQueue_.empty() && prec != diracPrec_)
       errorQcu("precision changed while right-hand sides are queued.\n");
-    delete dirac_;
-    dirac_ = new DiracWilson(soaGauge(prec), mass, nColors, floatPrecision, lattDesc_, procDesc_, dslashType,
-                             streams());
     diracPrec_ = prec;
+    diracType_ = dslashType;
+    diracMass_ = mass;
+    diracFloatPrec_ = floatPrecision;
+    makeDirac();
   }
 
   void pushFermion(void *fermionOut, void *fermionIn)
@@ -247,11 +267,22 @@ void DiracWilson::Dslash(void *outputMRHS, void *inputMRHS, int parity, int dagg
                     floatPrecision_, nColors_, nInputs, streams_);
 }
 
-void instantiateDslash(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &procDesc, int parity, int daggerFlag,
-                       double kappa, void *mrhsFermionIn, void *mrhsFermionOut, void *gauge, DSLASH_TYPE dtype,
-                       QCU_PRECISION floatPrecision, int nColors, int nInputs, ExecutionStreams &streams)
+void DiracWilson::DslashXpay(void *outputMRHS, void *inputMRHS, void *xMRHS, double a, double b, int parity,
+                             int daggerFlag, int nInputs)
 {
-  (void)kappa;  // the hopping term carries no kappa (reference qcu_wilson.cu:21-45)
+  instantiateDslashXpay(lattDesc_, procDesc_, parity, daggerFlag, a, b, inputMRHS, xMRHS, outputMRHS, gauge_,
+                        dslashType_, floatPrecision_, nColors_, nInputs, streams_);
+}
+
+namespace {
+
+// Shared by instantiateDslash and instantiateDslashXpay: reference SoA
+// pointer packs, synchronous on the operator's stream.
+void run_soa_dslash(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &procDesc, int parity, int daggerFlag,
+                    const void *const *x, double a, double b, void *mrhsFermionIn, void *mrhsFermionOut, void *gauge,
+                    DSLASH_TYPE dtype, QCU_PRECISION floatPrecision, int nColors, int nInputs,
+                    ExecutionStreams &streams)
+{
   if (dtype != DSLASH_WILSON) errorQcu("Now we only support Wilson dslash.\n");
   const int prec = to_prec(floatPrecision);
   if (prec < 0) errorQcu("Now we only support half, single and double precision.\n");
@@ -271,12 +302,37 @@ void instantiateDslash(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &p
   c.dagger = daggerFlag ? 1 : 0;
   c.out_ptrs = static_cast<void *const *>(mrhsFermionOut);
   c.in_ptrs = static_cast<const void *const *>(mrhsFermionIn);
+  c.x_ptrs = x;
+  c.xa = a;
+  c.xb = b;
   c.gauge = gauge;
   c.stream = streams.stream;
   std::string err;
   if (qmrhs::launch_dslash(c, &err)) fatal(err);
   CHECK_HIP(hipStreamSynchronize(streams.stream));
 }
+
+}  // namespace
+
+void instantiateDslashXpay(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &procDesc, int parity,
+                           int daggerFlag, double a, double b, void *mrhsFermionIn, void *mrhsFermionX,
+                           void *mrhsFermionOut, void *gauge, DSLASH_TYPE dtype, QCU_PRECISION floatPrecision,
+                           int nColors, int nInputs, ExecutionStreams &streams)
+{
+  if (!mrhsFermionX) errorQcu("DslashXpay: null xpay operand.\n");
+  run_soa_dslash(lattDesc, procDesc, parity, daggerFlag, static_cast<const void *const *>(mrhsFermionX), a, b,
+                 mrhsFermionIn, mrhsFermionOut, gauge, dtype, floatPrecision, nColors, nInputs, streams);
+}
+
+void instantiateDslash(const QcuLattDesc<Nd> &lattDesc, const QcuProcDesc<Nd> &procDesc, int parity, int daggerFlag,
+                       double kappa, void *mrhsFermionIn, void *mrhsFermionOut, void *gauge, DSLASH_TYPE dtype,
+                       QCU_PRECISION floatPrecision, int nColors, int nInputs, ExecutionStreams &streams)
+{
+  (void)kappa;  // the hopping term carries no kappa (reference qcu_wilson.cu:21-45)
+  run_soa_dslash(lattDesc, procDesc, parity, daggerFlag, nullptr, 0.0, 1.0, mrhsFermionIn, mrhsFermionOut, gauge,
+                 dtype, floatPrecision, nColors, nInputs, streams);
+}
+
 
 template <typename F> struct PrecOf;
 template <> struct PrecOf<double> { static constexpr QCU_PRECISION v = QCU_DOUBLE_PRECISION; };
