@@ -497,3 +497,45 @@ def test_xcd_cyclic_order(lib, dev):
             outs.append(out)
         torch.cuda.synchronize()
         assert bool(torch.equal(outs[0], outs[1])), (tb, zb, block)
+
+
+@pytest.mark.parametrize("in_prec,out_prec", [(_lib.HALF, _lib.HALF), (_lib.HALF, _lib.SINGLE),
+                                              (_lib.BF16, _lib.BF16), (_lib.BF16, _lib.SINGLE),
+                                              (_lib.SINGLE, _lib.SINGLE), (_lib.DOUBLE, _lib.DOUBLE)])
+def test_dslash_ex_scaled_without_xpay(lib, dev, in_prec, out_prec):
+    """qcuMrhsDslashEx with x = NULL: stored out = b D(sc_in * in) / sc_out and
+    |b D(sc_in * in)|^2 per RHS, at 16 RHS (the fixed-RHS-count kernels; for
+    16-bit input the default tile stages the per-RHS epilogue factors in LDS
+    beside the links), against the FP64 oracle."""
+    L, N, k = (4, 4, 4, 8), 3, 16
+    g = fields.gauge_field(L, N, seed=33)
+    psi = fields.gaussian_spinors(L, N, k, seed=34)
+    b = -0.135 ** 2
+    rng = np.random.default_rng(35)
+    si = (2.0 ** rng.integers(-3, 4, size=k)).astype(np.float32)
+    so = (2.0 ** rng.integers(-4, 3, size=k) * 0.75).astype(np.float32)
+    geom = _lib.geom(L, N)
+    gd = dev_tensor(g, dev)
+    gnat = empty_prec((lib.qcuMrhsGaugeBytes(geom, in_prec) // _lib.BYTES[in_prec],), in_prec, dev)
+    _lib.check(lib.qcuMrhsPackGauge(gnat.data_ptr(), in_prec, gd.data_ptr(), _lib.DOUBLE, geom, stream()))
+    conv = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}
+    pin = dev_tensor(fields.to_packed(psi / si[:, None, None, None]), dev).to(conv[in_prec])
+    pout = empty_prec(pin.shape[:-1], out_prec, dev)
+    dsi, dso = (torch.from_numpy(v).to(dev) for v in (si, so))
+    nrm = torch.zeros(k, dtype=torch.float64, device=dev)
+    for parity, dagger in ((0, 1), (1, 0)):
+        _lib.check(lib.qcuMrhsDslashEx(pout.data_ptr(), out_prec, pin.data_ptr(), None, in_prec, 0.0, b,
+                                       dsi.data_ptr(), None, dso.data_ptr(), nrm.data_ptr(), gnat.data_ptr(), k,
+                                       parity, dagger, geom, stream()))
+        torch.cuda.synchronize()
+        got = fields.from_packed(host_complex(pout)) * so[:, None, None, None]
+        psi_r = fields.from_packed(host_complex(pin)) * si[:, None, None, None]
+        want = b * oracle.dslash(g, psi_r, L, parity, dagger)
+        tol = {_lib.HALF: 3e-3, _lib.BF16: 2e-2, _lib.SINGLE: 1e-6, _lib.DOUBLE: 1e-13}
+        t = max(tol[in_prec], tol[out_prec])
+        err = rel_err(got, want)
+        wn = np.array([np.vdot(want[r], want[r]).real for r in range(k)])
+        nerr = float(np.max(np.abs(nrm.cpu().numpy() - wn) / wn))
+        print(f"in={in_prec} out={out_prec} parity={parity}: rel_err={err:.2e}, norm rel_err={nerr:.2e}")
+        assert err < t
+        assert nerr < 4 * t
