@@ -196,7 +196,10 @@ int qcuMrhsDslashFaces(void *out, const void *const *from_bwd, const void *const
  * records the fastest for (N, nrhs, prec, lattice) and returns it. With
  * QCU_MRHS_TUNE_CACHE=<file> in the environment, results persist in that text
  * file (keyed by GPU architecture) and are reused by later processes. With
- * QCU_MRHS_TUNE_LOG=1 every candidate tile and its time go to stderr. */
+ * QCU_MRHS_TUNE_LOG=1 every candidate tile and its time go to stderr.
+ * Shapes listed in the pinned tile table (tiles_<arch>.txt beside the
+ * library, or the file named by QCU_MRHS_TILES; same line format) count as
+ * tuned: the dispatcher and the solver use those tiles without tuning. */
 int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *g,
                 qcuMrhsTile *best, double *best_us, void *stream);
 /* Tile the dispatcher uses for a shape (tuned if known, else the default). */

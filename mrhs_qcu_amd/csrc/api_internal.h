@@ -27,4 +27,15 @@ int run_packed_dslash(void *out, const void *in, const void *x, double xa, doubl
                       int k, int parity, int dagger, const Geom &g, const Tile *tile, hipStream_t st,
                       std::string *err);
 
+// qcuMrhsDslashEx in three steps, shared with its allocation-free variant:
+// the call (tuned tile for the shape), the bytes of norm partials it needs
+// (per-workgroup partials and the reduction's stage buffer), and the launch
+// plus the per-RHS reduction into norm2_out (when not null) using `scratch`
+// of at least that size. Asynchronous on the call's stream.
+bool dslash_ex_call(DslashCall *c, void *out, int out_prec, const void *in, const void *x, int in_prec, double a,
+                    double b, const float *sc_in, const float *sc_x, const float *sc_out, const void *gauge, int nrhs,
+                    int parity, int dagger, const qcuMrhsGeom *gc, void *stream, std::string *err);
+size_t dslash_ex_scratch_bytes(const DslashCall &c);
+int dslash_ex_launch(const DslashCall &c, double *norm2_out, double *scratch, std::string *err);
+
 }  // namespace qmrhs

@@ -2,6 +2,7 @@
 // the tile autotuner.
 #include "qcu_mrhs.h"
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -87,10 +88,26 @@ std::string device_arch()
 
 const char *tune_file() { return getenv("QCU_MRHS_TUNE_CACHE"); }
 
-void load_tune_file(std::map<TuneKey, Tile> &m)
+// The pinned tile table: QCU_MRHS_TILES, or tiles_<arch>.txt in the directory
+// of this library (tiles the autotuner chose on that GPU for the benchmark and
+// solver shapes, shipped with the library so that every process uses the same
+// tile for them instead of re-tuning near-ties differently each run).
+std::string pinned_tile_file(const std::string &arch)
 {
-  const char *path = tune_file();
-  if (!path || !path[0]) return;
+  const char *env = getenv("QCU_MRHS_TILES");
+  if (env) return env;
+  Dl_info info;
+  if (!dladdr(reinterpret_cast<void *>(&qcuMrhsVersion), &info) || !info.dli_fname) return "";
+  std::string dir = info.dli_fname;
+  const size_t slash = dir.rfind('/');
+  dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+  return dir + "/tiles_" + arch + ".txt";
+}
+
+void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
+{
+  const char *path = path_s.c_str();
+  if (!path[0]) return;
   FILE *f = fopen(path, "r");
   if (!f) return;
   const std::string arch = device_arch();
@@ -131,7 +148,9 @@ std::map<TuneKey, Tile> &tune_cache()
   static bool loaded = false;
   if (!loaded) {
     loaded = true;
-    load_tune_file(m);
+    load_tune_file(m, pinned_tile_file(device_arch()));  // later lines and the cache file win
+    const char *cache = tune_file();
+    if (cache) load_tune_file(m, cache);
   }
   return m;
 }
@@ -266,6 +285,57 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.occupancy = t.occ;
   c.traversal = t.ystream;
   return c;
+}
+
+bool dslash_ex_call(DslashCall *cp, void *out, int out_prec, const void *in, const void *x, int in_prec, double a,
+                    double b, const float *sc_in, const float *sc_x, const float *sc_out, const void *gauge, int nrhs,
+                    int parity, int dagger, const qcuMrhsGeom *gc, void *stream, std::string *err)
+{
+  Geom g;
+  if (!geom_from_c(gc, &g, err)) return false;
+  DslashCall &c = *cp;
+  c.sl = SL_PACKED;
+  c.gl = GL_NATIVE;
+  c.prec_s = c.prec_g = in_prec;
+  c.prec_o = out_prec;
+  c.geom = g;
+  c.tile = tile_for(g, in_prec, nrhs);
+  c.k = nrhs;
+  c.parity = parity;
+  c.dagger = dagger;
+  c.out = out;
+  c.in = in;
+  c.x = x;
+  c.xa = a;
+  c.xb = b;
+  c.gauge = gauge;
+  c.sc_in = sc_in;
+  c.sc_x = sc_x;
+  c.sc_out = sc_out;
+  c.stream = static_cast<hipStream_t>(stream);
+  return true;
+}
+
+size_t dslash_ex_scratch_bytes(const DslashCall &c)
+{
+  // [nblocks][nrhs] per-workgroup partials followed by the reduction's stage buffer
+  return (size_t(dslash_num_blocks(c)) + kReduceStages) * size_t(c.k) * sizeof(double);
+}
+
+int dslash_ex_launch(const DslashCall &call, double *norm2_out, double *scratch, std::string *err)
+{
+  DslashCall c = call;
+  const long long nb = dslash_num_blocks(c);
+  c.partial = norm2_out ? scratch : nullptr;
+  if (launch_dslash(c, err)) return 1;
+  if (norm2_out) {
+    const hipError_t e = blas_reduce(scratch, nb, c.k, norm2_out, scratch + size_t(nb) * c.k, c.stream);
+    if (e != hipSuccess) {
+      if (err) *err = hipGetErrorString(e);
+      return 1;
+    }
+  }
+  return 0;
 }
 
 int run_packed_dslash(void *out, const void *in, const void *x, double xa, double xb, const void *gauge, int prec,
@@ -431,45 +501,19 @@ int qcuMrhsDslashEx(void *out, int out_prec, const void *in, const void *x, int 
                     const float *sc_in, const float *sc_x, const float *sc_out, double *norm2_out, const void *gauge,
                     int nrhs, int parity, int dagger, const qcuMrhsGeom *gc, void *stream)
 {
-  Geom g;
   std::string err;
-  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
-  hipStream_t st = static_cast<hipStream_t>(stream);
   DslashCall c;
-  c.sl = SL_PACKED;
-  c.gl = GL_NATIVE;
-  c.prec_s = c.prec_g = in_prec;
-  c.prec_o = out_prec;
-  c.geom = g;
-  c.tile = tile_for(g, in_prec, nrhs);
-  c.k = nrhs;
-  c.parity = parity;
-  c.dagger = dagger;
-  c.out = out;
-  c.in = in;
-  c.x = x;
-  c.xa = a;
-  c.xb = b;
-  c.gauge = gauge;
-  c.sc_in = sc_in;
-  c.sc_x = sc_x;
-  c.sc_out = sc_out;
-  c.stream = st;
+  if (!dslash_ex_call(&c, out, out_prec, in, x, in_prec, a, b, sc_in, sc_x, sc_out, gauge, nrhs, parity, dagger, gc,
+                      stream, &err))
+    return api_fail(err);
   double *partial = nullptr;
-  const long long nb = dslash_num_blocks(c);
-  if (norm2_out) {
-    // [nb][nrhs] partials followed by the reduction's stage buffer
-    if (hipMalloc(&partial, (size_t(nb) + kReduceStages) * nrhs * sizeof(double)) != hipSuccess)
-      return api_fail("allocation failed");
-    c.partial = partial;
+  if (norm2_out && hipMalloc(&partial, dslash_ex_scratch_bytes(c)) != hipSuccess) return api_fail("allocation failed");
+  int rc = dslash_ex_launch(c, norm2_out, partial, &err);
+  // The partials are freed on return, so the call completes first.
+  if (partial) {
+    if (hipStreamSynchronize(c.stream) != hipSuccess && !rc) rc = 1, err = "synchronisation failed";
+    (void)hipFree(partial);
   }
-  int rc = launch_dslash(c, &err);
-  if (!rc && norm2_out) {
-    hipError_t e = blas_reduce(partial, nb, nrhs, norm2_out, partial + size_t(nb) * nrhs, st);
-    if (e != hipSuccess) rc = 1, err = hipGetErrorString(e);
-    if (hipStreamSynchronize(st) != hipSuccess) rc = 1, err = "synchronisation failed";
-  }
-  if (partial) (void)hipFree(partial);
   if (rc) return api_fail(err);
   return 0;
 }
