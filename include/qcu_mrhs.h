@@ -98,6 +98,18 @@ int qcuMrhsDslashXpay(void *out, const void *in, const void *x, double a, double
 int qcuMrhsDslashEx(void *out, int out_prec, const void *in, const void *x, int in_prec, double a, double b,
                     const float *sc_in, const float *sc_x, const float *sc_out, double *norm2_out, const void *gauge,
                     int nrhs, int parity, int dagger, const qcuMrhsGeom *g, void *stream);
+/* No counterpart in the reference. qcuMrhsDslashEx without any device
+ * allocation, for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation): the same
+ * arguments and kernels, with the per-RHS norm partials in a fixed 16 MiB
+ * device array the library holds on each GPU until the process ends. Calls
+ * whose norms need more (workgroups + 256) * nrhs * 8 bytes than that fail
+ * with a message. Asynchronous on `stream`. Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other streams, other host
+ * threads) race; calls in one stream are ordered. */
+int qcuMrhsDslashExNoAlloc(void *out, int out_prec, const void *in, const void *x, int in_prec, double a, double b,
+                           const float *sc_in, const float *sc_x, const float *sc_out, double *norm2_out,
+                           const void *gauge, int nrhs, int parity, int dagger, const qcuMrhsGeom *g, void *stream);
 /* Hopping term on nrhs user spinors (host arrays of device pointers), native
  * gauge in prec; the qcu.h startDslash path without the queue. */
 int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const void *gauge, int prec, int nrhs,

@@ -58,7 +58,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
                       "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
                       "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
-                      "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth"]
+                      "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth", "qcuMrhsDslashExNoAlloc"]
 
 _lib = None
 
@@ -94,6 +94,7 @@ def _declare(lib):
         "qcuMrhsDslashXpay": (I, [P, P, P, D, D, P, I, I, I, I, G, T, P]),
         "qcuMrhsDslashUser": (I, [PP, PP, P, I, I, I, I, G, P]),
         "qcuMrhsDslashEx": (I, [P, I, P, P, I, D, D, P, P, P, P, P, I, I, I, G, P]),
+        "qcuMrhsDslashExNoAlloc": (I, [P, I, P, P, I, D, D, P, P, P, P, P, I, I, I, G, P]),
         "qcuMrhsDslashReferenceLayout": (I, [PP, PP, P, I, I, I, I, G]),
         "qcuMrhsTune": (I, [P, P, P, I, I, G, T, ctypes.POINTER(D), P]),
         "qcuMrhsGetTile": (I, [G, I, I, T]),
