@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 _GAUGE_CACHE = {}
+RETUNE = False  # --retune: tune the timed shapes instead of using the pinned tiles
 
 
 def bytes_alg(S, k, N, b_in, b_out, b_u):
@@ -55,6 +56,8 @@ def parse():
     ap.add_argument("--cg-solver", choices=["both", "fp64", "mixed"], default="both",
                     help="with --cg-only: run one solver only (profiling)")
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
+    ap.add_argument("--retune", action="store_true",
+                    help="tune the timed shapes instead of using the pinned tiles shipped with the library")
     ap.add_argument("--tile", type=str, default=None,
                     help="fixed tile R,block,lds,tblock[,zblock,hints,occupancy,traversal] for the headline kernel "
                          "(skips the tuner)")
@@ -91,16 +94,24 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
-    # Two fixed Gaussian inputs and two outputs: step i applies the operator of
-    # parity i & 1 to ins[i & 1], so every timed launch sees the same finite
-    # data (feeding outputs back in would grow the field ~4x per application).
-    ins = [torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt) for _ in range(2)]
-    outs = [torch.empty_like(ins[0]) for _ in range(2)]
+    # One fixed Gaussian input and one output: step i applies the operator of
+    # parity i & 1 to the same input, so every timed launch sees the same
+    # finite data (feeding outputs back in would grow the field ~4x per
+    # application) and the fields touched are the two a ping-pong would touch.
+    a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
+    ins = [a, a]
+    o = torch.empty_like(a)
+    outs = [o, o]
     if tile is None:
+        # The pinned tile of this shape (tiles_<arch>.txt beside the library),
+        # else the default; --retune times every candidate here instead.
         best = _lib.Tile()
-        us = ctypes.c_double()
-        _lib.check(lib.qcuMrhsTune(outs[0].data_ptr(), ins[0].data_ptr(), gn.data_ptr(), prec, k, geom,
-                                   ctypes.byref(best), ctypes.byref(us), sp))
+        if RETUNE:
+            us = ctypes.c_double()
+            _lib.check(lib.qcuMrhsTune(outs[0].data_ptr(), ins[0].data_ptr(), gn.data_ptr(), prec, k, geom,
+                                       ctypes.byref(best), ctypes.byref(us), sp))
+        else:
+            _lib.check(lib.qcuMrhsGetTile(geom, prec, k, ctypes.byref(best)))
         tp = None
     else:
         best = _lib.tile(*tile)
@@ -143,8 +154,10 @@ def time_dslash12(lib, _lib, torch, geom, prec, k, steps, warmup, L, seed=0, rea
     g12 = torch.empty(nbytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
     # The generator folds antiperiodic time into the links: t_sign = -1.
     _lib.check(pack(g12.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, -1, sp))
-    ins = [torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt) for _ in range(2)]
-    outs = [torch.empty_like(ins[0]) for _ in range(2)]
+    a = torch.randn(S * 4 * 3 * k * 2, device=dev, dtype=torch.float32).to(dt)
+    ins = [a, a]
+    o = torch.empty_like(a)
+    outs = [o, o]
     args = [(outs[p].data_ptr(), ins[p].data_ptr(), g12.data_ptr(), prec, k, p, 0, -1, geom, None, sp) for p in (0, 1)]
     for i in range(warmup):
         _lib.check(run(*args[i & 1]))
@@ -274,7 +287,8 @@ def faces_benchmark(lib, _lib, torch, L=(24, 24, 24, 24), grid=(1, 1, 1, 2), N=3
     del gu
     a = torch.randn(S * 4 * N * k * 2, device=dev, dtype=torch.float32)
     b = torch.empty_like(a)
-    _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, None, None, sp))
+    if RETUNE:
+        _lib.check(lib.qcuMrhsTune(b.data_ptr(), a.data_ptr(), gn.data_ptr(), prec, k, geom, None, None, sp))
     tb, tf = [0] * 4, [0] * 4
     keep = []
     pack_bytes = faces_bytes = 0
@@ -527,6 +541,8 @@ def traffic_from_profile(name):
 
 def main():
     args = parse()
+    global RETUNE
+    RETUNE = args.retune
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               f"(the multi-GPU sharding is design-only)", file=sys.stderr)
@@ -572,6 +588,8 @@ def main():
     achieved = ba / (ms * 1e-3) / 1e9
     wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_{args.precision}"
     traffic, traffic_tile = traffic_from_profile(f"{wl}_traffic.json")
+    if traffic_tile != ",".join(str(v) for v in tile):
+        traffic = None  # measured on another tile: not evidence for this one
     line = {
         "metric": "MRHS dslash sites\u00b7RHS/s + %HBM roofline; mixed vs FP64 CG time to 1e-10",
         "value": value,

@@ -111,10 +111,13 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
   FILE *f = fopen(path, "r");
   if (!f) return;
   const std::string arch = device_arch();
-  char a[64];
+  char line[512], a[64];
   int v[17];
-  while (fscanf(f, "%63s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", a, &v[0], &v[1], &v[2], &v[3], &v[4],
-                &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13], &v[14], &v[15]) == 17) {
+  // One shape per line; lines that do not parse (comments) are skipped.
+  while (fgets(line, sizeof line, f)) {
+    if (sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d", a, &v[0], &v[1], &v[2], &v[3], &v[4],
+               &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13], &v[14], &v[15]) != 17)
+      continue;
     if (arch != a) continue;
     Tile t;
     t.R = v[8];
@@ -236,9 +239,9 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
           }
         }
         // Blocks streamed along y (t_block x z_block cross-sections).
-        for (int tb : {2, 3, 4, 6, 8}) {
+        for (int tb : {2, 3, 4, 6, 8, 12, 16}) {
           if (g.L[3] % tb || tb >= g.L[3]) continue;
-          for (int zb : {2, 3, 4, 6, 8}) {
+          for (int zb : {2, 3, 4, 6, 8, 12, 16}) {
             if (g.L[2] % zb || zb >= g.L[2]) continue;
             Tile t;
             t.R = R;

@@ -272,3 +272,24 @@ def test_tune_cache_file_is_keyed_by_architecture_and_shape(tmp_path):
     p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     default = json.loads(p.stdout.strip().splitlines()[-1])
     assert tuple(got[2]) == tuple(default[2])  # shape not in the file: default tile
+
+
+def test_pinned_tile_table():
+    """The tile table shipped beside the library: every shape line has the
+    17 fields of the tune-cache format, a launchable tile (R divides nrhs,
+    block a multiple of 64 up to 256) and lists the benchmark and solver
+    shapes (24^3x48 and 32^3x64, 16 RHS, SU(3), every precision)."""
+    path = os.path.join(ROOT, "mrhs_qcu_amd", "tiles_gfx950.txt")
+    shapes = set()
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split()
+        assert len(f) == 17 and f[0] == "gfx950", line
+        N, k, prec, Lx, Ly, Lz, Lt, mask, R, block, lds, tb, zb, hints, occ, trav = map(int, f[1:])
+        assert k % R == 0 and block in (64, 128, 192, 256) and lds in (0, 1), line
+        assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
+        shapes.add((N, k, prec, (Lx, Ly, Lz, Lt), mask))
+    for L in ((24, 24, 24, 48), (32, 32, 32, 64)):
+        for prec in (0, 1, 2, 4):
+            assert (3, 16, prec, L, 0) in shapes, (L, prec)

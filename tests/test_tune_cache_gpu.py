@@ -56,3 +56,34 @@ def test_tune_cache_file_round_trip(tmp_path):
     env.pop("QCU_MRHS_TUNE_CACHE")
     default = run_child("get", env)
     assert len(default) == len(tuned)
+
+
+PINNED = r"""
+import ctypes, json, sys
+sys.path.insert(0, %(root)r)
+import torch
+from mrhs_qcu_amd import _lib
+lib = _lib.load()
+torch.zeros(1, device="cuda:0")
+t = _lib.Tile()
+_lib.check(lib.qcuMrhsGetTile(_lib.geom((24, 24, 24, 48), 3), _lib.SINGLE, 16, ctypes.byref(t)))
+print(json.dumps(t.as_tuple()))
+"""
+
+
+def test_pinned_tiles_used_without_tuning():
+    """A fresh process without tuning gets the headline shape's tile from the
+    table shipped beside the library (tiles_gfx950.txt)."""
+    want = None
+    for line in open(os.path.join(ROOT, "mrhs_qcu_amd", "tiles_gfx950.txt")):
+        f = line.split()
+        if len(f) == 17 and f[0] == "gfx950" and f[1:9] == ["3", "16", "1", "24", "24", "24", "48", "0"]:
+            want = tuple(int(v) for v in f[9:])
+    assert want is not None
+    env = dict(os.environ)
+    env.pop("QCU_MRHS_TILES", None)
+    env.pop("QCU_MRHS_TUNE_CACHE", None)
+    p = subprocess.run([sys.executable, "-c", PINNED % {"root": ROOT}], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert p.returncode == 0, p.stderr[-2000:]
+    assert tuple(json.loads(p.stdout.strip().splitlines()[-1])) == want
