@@ -272,9 +272,15 @@ typedef struct qcuMrhsProblem_s {
 } qcuMrhsProblem;
 
 /* Builds one replayable launch (a HIP graph) applying the dslash to every
- * problem, with a tuned tile per problem. */
+ * problem, with a tuned tile per problem. With every nColors in 2..5 the
+ * graph is a single work-list kernel covering all problems (per-N tile
+ * family: 2 RHS per lane with links staged in LDS for N <= 4, 1 RHS per lane
+ * for N = 5; 128 lanes per workgroup; each problem's tuned traversal);
+ * otherwise one kernel per problem. */
 int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, void *stream);
 int qcuMrhsBatchRun(void *handle, void *stream);
+/* The tile the batch runs problem i with (the caller's problem order). */
+int qcuMrhsBatchGetTile(void *handle, int i, qcuMrhsTile *tile);
 int qcuMrhsBatchDestroy(void *handle);
 
 #ifdef __cplusplus

@@ -53,7 +53,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsPackGauge", "qcuMrhsPackSpinors", "qcuMrhsUnpackSpinors", "qcuMrhsConvert",
                       "qcuMrhsShiftSpinor", "qcuMrhsShiftGauge", "qcuMrhsDslash", "qcuMrhsDslashXpay", "qcuMrhsDslashEx",
                       "qcuMrhsDslashUser", "qcuMrhsDslashReferenceLayout", "qcuMrhsTune", "qcuMrhsGetTile",
-                      "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun",
+                      "qcuMrhsKernelCount", "qcuMrhsCgSolve", "qcuMrhsSolverRelease", "qcuMrhsBatchCreate", "qcuMrhsBatchRun", "qcuMrhsBatchGetTile",
                       "qcuMrhsBatchDestroy", "qcuMrhsFaceBytes", "qcuMrhsPackFaces", "qcuMrhsDslashFaces",
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
                       "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
@@ -103,6 +103,7 @@ def _declare(lib):
         "qcuMrhsSolverRelease": (I, []),
         "qcuMrhsBatchCreate": (I, [ctypes.POINTER(Problem), I, ctypes.POINTER(P), P]),
         "qcuMrhsBatchRun": (I, [P, P]),
+        "qcuMrhsBatchGetTile": (I, [P, I, T]),
         "qcuMrhsBatchDestroy": (I, [P]),
         "qcuMrhsFaceBytes": (ctypes.c_size_t, [G, I, I, I]),
         "qcuMrhsPackFaces": (I, [PP, PP, P, P, I, I, I, I, G, P]),

@@ -2,13 +2,16 @@
 // count N, nrhs, parity, fields) applied by one replayable HIP graph.
 //
 // Create tunes every problem shape that has no tile yet (timing it on the
-// problem's own fields, so `out` is scratch until the first Run), freezes the
-// tile of each problem and captures one kernel node per problem, largest
-// first, as a chain: each launch already fills the GPU (>= 256 workgroups for
-// every shape of interest), and independent branches measured slower than the
-// chain. Run is a single hipGraphLaunch. The graph holds the field pointers
-// given at Create: the caller keeps those buffers alive and unmoved until
-// Destroy. Nothing in the graph clears memory (no memset nodes).
+// problem's own fields, so `out` is scratch until the first Run) and freezes
+// one tile per problem. When every problem has N = 2..5 (the colour counts
+// the work-list kernel is compiled for), the graph holds ONE kernel node: a
+// work-list launch whose workgroups cover all problems, largest first, each
+// problem in its own range with its own tile traversal (batch_kernels.h), so
+// the small problems run in the tail of the large ones instead of after it.
+// Otherwise the graph chains one kernel node per problem, largest first.
+// Run is a single hipGraphLaunch. The graph holds the field pointers given at
+// Create: the caller keeps those buffers alive and unmoved until Destroy.
+// Nothing in the graph clears memory (no memset nodes).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +19,7 @@
 #include <vector>
 
 #include "api_internal.h"
+#include "batch_kernels.h"
 #include "qcu_mrhs.h"
 
 namespace qmrhs {
@@ -26,10 +30,12 @@ struct Batch {
   std::vector<Tile> tiles;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  BatchItem *items = nullptr;  // device copy of the work list (work-list graphs)
   ~Batch()
   {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (items) (void)hipFree(items);
   }
 };
 
@@ -104,6 +110,23 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
     b->probs.push_back(p);
     b->tiles.push_back(tile_for(geoms[i], PREC_SINGLE, p.nrhs));
   }
+  // The work list needs every problem in the compiled tile family (RHS per
+  // lane and link staging per N) with one workgroup size; the traversal of
+  // each problem's tuned tile is kept.
+  bool worklist = true;
+  for (int i = 0; i < nprob && worklist; ++i) {
+    int R = 0, lds = 0;
+    worklist = batch_body_tile(geoms[i].N, &R, &lds) && b->probs[i].nrhs % R == 0;
+    if (worklist) {
+      Tile &t = b->tiles[i];
+      t.R = R;
+      t.lds = lds;
+      t.block = kBatchBlock;
+      t.occ = 0;
+    }
+  }
+  if (!worklist)
+    for (int i = 0; i < nprob; ++i) b->tiles[i] = tile_for(geoms[i], PREC_SINGLE, b->probs[i].nrhs);
   // Every launch is validated eagerly first, so a bad problem fails here and
   // not inside the capture.
   for (int i = 0; i < nprob; ++i) {
@@ -131,11 +154,39 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       delete b;
       return api_fail("batch: stream creation failed");
     }
-    ok = hipStreamBeginCapture(cs.cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-    cs.capturing = ok;
-    for (int j = 0; ok && j < nprob; ++j) {
+    // Work list: the problems' ranges in launch order, each starting at a
+    // multiple of the XCD count; the dynamic LDS is the largest request.
+    std::vector<BatchItem> items;
+    uint32_t total = 0;
+    size_t shm_max = 0;
+    for (int j = 0; worklist && ok && j < nprob; ++j) {
       const int i = order[j];
-      ok = !launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.cap), &err);
+      BatchItem it{};
+      size_t shm = 0;
+      if (packed_launch_args(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.cap), &it.a, &shm, &err)) {
+        ok = false;
+        break;
+      }
+      it.first = total;
+      it.n = geoms[i].N;
+      total += (it.a.nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+      shm_max = std::max(shm_max, shm);
+      items.push_back(it);
+    }
+    if (worklist && ok) {
+      ok = hipMalloc(&b->items, items.size() * sizeof(BatchItem)) == hipSuccess &&
+           hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess;
+      if (!ok) err = "work-list allocation failed";
+    }
+    ok = ok && hipStreamBeginCapture(cs.cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    cs.capturing = ok;
+    if (worklist) {
+      if (ok) ok = launch_batch_worklist(b->items, int(items.size()), total, shm_max, cs.cap) == hipSuccess;
+    } else {
+      for (int j = 0; ok && j < nprob; ++j) {
+        const int i = order[j];
+        ok = !launch_dslash(problem_call(b->probs[i], geoms[i], b->tiles[i], cs.cap), &err);
+      }
     }
     hipGraph_t g = nullptr;
     if (cs.capturing) {
@@ -164,6 +215,15 @@ int qcuMrhsBatchRun(void *handle, void *stream)
   Batch *b = static_cast<Batch *>(handle);
   const hipError_t e = hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return api_fail(std::string("batch launch failed: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int qcuMrhsBatchGetTile(void *handle, int i, qcuMrhsTile *tile)
+{
+  if (!handle || !tile) return api_fail("null batch handle or tile");
+  Batch *b = static_cast<Batch *>(handle);
+  if (i < 0 || i >= int(b->tiles.size())) return api_fail("problem index out of range");
+  *tile = tile_to_c(b->tiles[i]);
   return 0;
 }
 

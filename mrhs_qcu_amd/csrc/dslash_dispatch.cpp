@@ -178,7 +178,76 @@ unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k)
   return unsigned(gran);
 }
 
-int launch_dslash(const DslashCall &c, std::string *err)
+int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchFn *fn_out, std::string *err)
+{
+  DslashArgs &a = *ap;
+  const int B = c.tile.block;
+  const int R = c.tile.R;
+  if (R != 1 && R != 2 && R != 4) return fail(err, "rhs_per_lane must be 1, 2 or 4"), 1;
+  if (c.k % R) return fail(err, "nrhs must be a multiple of rhs_per_lane"), 1;
+  if (!c.out || !c.in) return fail(err, "null spinor"), 1;
+  const uintptr_t al = uintptr_t(16);
+  if ((uintptr_t(c.out) | uintptr_t(c.in) | uintptr_t(c.gauge) | uintptr_t(c.x)) % al)
+    return fail(err, "packed fields must be 16-byte aligned"), 1;
+  if (c.partial && c.k > 256) return fail(err, "norm partials support at most 256 RHS"), 1;
+  if (c.rx_r) {
+    if (c.prec_o != PREC_SINGLE && c.prec_o != PREC_DOUBLE)
+      return fail(err, "the fused CG update needs FP32 or FP64 output"), 1;
+    if (!c.rx_alpha) return fail(err, "fused CG update: null alpha"), 1;
+    if (uintptr_t(c.rx_r) % al) return fail(err, "packed fields must be 16-byte aligned"), 1;
+  }
+  if (c.gl != GL_NATIVE && c.gl != GL_NATIVE12 && c.gl != GL_NATIVE8)
+    return fail(err, "packed spinors use the native (or two-row / eight-real SU(3)) gauge layout"), 1;
+  if (c.gl == GL_NATIVE12 && c.geom.N != 3) return fail(err, "two-row links need nColors = 3"), 1;
+  if (c.gl == GL_NATIVE8 && c.geom.N != 3) return fail(err, "eight-real links need nColors = 3"), 1;
+  if (c.gl == GL_NATIVE8 && !c.tile.lds)
+    return fail(err, "eight-real links are rebuilt while staging: the tile needs link_lds = 1"), 1;
+  const int lds = c.tile.lds ? 1 : 0;
+  KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
+  // The same kernel compiled for this exact RHS count, when one exists
+  // (bit-identical results, constant row offsets).
+  KernelKey key_kf = key;
+  key_kf.kf = c.k;
+  LaunchFn fn = find_kernel(key_kf);
+  if (!fn) fn = find_kernel(key);
+  if (!fn) return fail(err, "no kernel instantiated for this precision / N / tile"), 1;
+  a.k = c.k;
+  a.P = c.k / R;
+  a.div_P = make_fastdiv(uint32_t(a.P));
+  const unsigned long long lanes = 1ULL * a.S * a.P;
+  if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
+  a.nlanes = uint32_t(lanes);
+  a.nblocks = uint32_t((lanes + B - 1) / B);
+  a.xcd_gran = xcd_cycle_gran(c.geom, c.tile, c.k);
+  size_t shm = 0;
+  if (lds) {
+    const size_t nsite_max = size_t((B - 1) / a.P + 2);
+    // Two-row and eight-real links are staged as full links (rebuilt while staging).
+    shm = nsite_max * 8 * link_cplx(GL_NATIVE, c.geom.N) * size_t(bytes_of(c.prec_g));
+    if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
+    // Per-RHS epilogue factors staged beside the links when they are used
+    // and fit, for 16-bit input (FP32 accumulation): at 32^3x64, 16 RHS,
+    // the scaled FP16 operator takes 624 instead of 644 us; for FP32 input
+    // it made no difference (1,094 / 1,097 us).
+    const bool half_in = c.prec_s == PREC_HALF || c.prec_s == PREC_BF16;
+    const size_t efac = size_t(4) * c.k * sizeof(float);
+    if (half_in && !a.plain_out && shm + efac <= size_t(kMaxDynLds)) {
+      a.efac_off = uint32_t(shm);
+      shm += efac;
+    }
+  }
+  if (c.tile.occ > 0) {
+    // Pad the LDS request so at most `occ` workgroups fit on a CU.
+    const size_t cap = size_t(kLdsPerCu / c.tile.occ) - 2048 - 512;
+    if (cap > size_t(kMaxDynLds)) return fail(err, "occupancy cap below 3 workgroups per CU is not reachable"), 1;
+    if (shm < cap) shm = cap;
+  }
+  *shm_out = shm;
+  *fn_out = fn;
+  return 0;
+}
+
+static int check_call(const DslashCall &c, std::string *err)
 {
   if (!check_geom(c.geom, err)) return 1;
   if (c.k < 1) return fail(err, "nrhs must be >= 1"), 1;
@@ -190,71 +259,31 @@ int launch_dslash(const DslashCall &c, std::string *err)
   if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
   if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
   if (!c.gauge) return fail(err, "null gauge"), 1;
+  return 0;
+}
+
+int packed_launch_args(const DslashCall &c, DslashArgs *a, size_t *shm, std::string *err)
+{
+  if (c.sl != SL_PACKED) return fail(err, "work lists hold packed-layout launches"), 1;
+  if (check_call(c, err)) return 1;
+  *a = DslashArgs{};
+  fill_common(*a, c);
+  LaunchFn fn = nullptr;
+  return prepare_packed(c, a, shm, &fn, err);
+}
+
+int launch_dslash(const DslashCall &c, std::string *err)
+{
+  if (check_call(c, err)) return 1;
+  const int B = c.tile.block;
 
   DslashArgs a{};
   fill_common(a, c);
 
   if (c.sl == SL_PACKED) {
-    const int R = c.tile.R;
-    if (R != 1 && R != 2 && R != 4) return fail(err, "rhs_per_lane must be 1, 2 or 4"), 1;
-    if (c.k % R) return fail(err, "nrhs must be a multiple of rhs_per_lane"), 1;
-    if (!c.out || !c.in) return fail(err, "null spinor"), 1;
-    const uintptr_t al = uintptr_t(16);
-    if ((uintptr_t(c.out) | uintptr_t(c.in) | uintptr_t(c.gauge) | uintptr_t(c.x)) % al)
-      return fail(err, "packed fields must be 16-byte aligned"), 1;
-    if (c.partial && c.k > 256) return fail(err, "norm partials support at most 256 RHS"), 1;
-    if (c.rx_r) {
-      if (c.prec_o != PREC_SINGLE && c.prec_o != PREC_DOUBLE)
-        return fail(err, "the fused CG update needs FP32 or FP64 output"), 1;
-      if (!c.rx_alpha) return fail(err, "fused CG update: null alpha"), 1;
-      if (uintptr_t(c.rx_r) % al) return fail(err, "packed fields must be 16-byte aligned"), 1;
-    }
-    if (c.gl != GL_NATIVE && c.gl != GL_NATIVE12 && c.gl != GL_NATIVE8)
-      return fail(err, "packed spinors use the native (or two-row / eight-real SU(3)) gauge layout"), 1;
-    if (c.gl == GL_NATIVE12 && c.geom.N != 3) return fail(err, "two-row links need nColors = 3"), 1;
-    if (c.gl == GL_NATIVE8 && c.geom.N != 3) return fail(err, "eight-real links need nColors = 3"), 1;
-    if (c.gl == GL_NATIVE8 && !c.tile.lds)
-      return fail(err, "eight-real links are rebuilt while staging: the tile needs link_lds = 1"), 1;
-    const int lds = c.tile.lds ? 1 : 0;
-    KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
-    // The same kernel compiled for this exact RHS count, when one exists
-    // (bit-identical results, constant row offsets).
-    KernelKey key_kf = key;
-    key_kf.kf = c.k;
-    LaunchFn fn = find_kernel(key_kf);
-    if (!fn) fn = find_kernel(key);
-    if (!fn) return fail(err, "no kernel instantiated for this precision / N / tile"), 1;
-    a.k = c.k;
-    a.P = c.k / R;
-    a.div_P = make_fastdiv(uint32_t(a.P));
-    const unsigned long long lanes = 1ULL * a.S * a.P;
-    if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
-    a.nlanes = uint32_t(lanes);
-    a.nblocks = uint32_t((lanes + B - 1) / B);
-    a.xcd_gran = xcd_cycle_gran(c.geom, c.tile, c.k);
     size_t shm = 0;
-    if (lds) {
-      const size_t nsite_max = size_t((B - 1) / a.P + 2);
-      // Two-row and eight-real links are staged as full links (rebuilt while staging).
-      shm = nsite_max * 8 * link_cplx(GL_NATIVE, c.geom.N) * size_t(bytes_of(c.prec_g));
-      if (shm > kMaxDynLds) return fail(err, "LDS link staging exceeds the per-workgroup LDS budget"), 1;
-      // Per-RHS epilogue factors staged beside the links when they are used
-      // and fit, for 16-bit input (FP32 accumulation): at 32^3x64, 16 RHS,
-      // the scaled FP16 operator takes 624 instead of 644 us; for FP32 input
-      // it made no difference (1,094 / 1,097 us).
-      const bool half_in = c.prec_s == PREC_HALF || c.prec_s == PREC_BF16;
-      const size_t efac = size_t(4) * c.k * sizeof(float);
-      if (half_in && !a.plain_out && shm + efac <= size_t(kMaxDynLds)) {
-        a.efac_off = uint32_t(shm);
-        shm += efac;
-      }
-    }
-    if (c.tile.occ > 0) {
-      // Pad the LDS request so at most `occ` workgroups fit on a CU.
-      const size_t cap = size_t(kLdsPerCu / c.tile.occ) - 2048 - 512;
-      if (cap > size_t(kMaxDynLds)) return fail(err, "occupancy cap below 3 workgroups per CU is not reachable"), 1;
-      if (shm < cap) shm = cap;
-    }
+    LaunchFn fn = nullptr;
+    if (prepare_packed(c, &a, &shm, &fn, err)) return 1;
     hipError_t e = fn(a, dim3(a.nblocks), dim3(B), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
     return 0;

@@ -104,6 +104,10 @@ struct DslashCall {
 
 // Returns 0 on success; otherwise a message in *err (when non-null).
 int launch_dslash(const DslashCall &c, std::string *err);
+// The kernel arguments and dynamic LDS bytes launch_dslash would use for a
+// packed-layout call (validated the same way), without launching; for
+// work-list launches that run several calls' workgroups in one kernel.
+int packed_launch_args(const DslashCall &c, DslashArgs *a, size_t *shm, std::string *err);
 // Workgroups a packed-layout launch uses (sizing of the partial-sum buffer).
 long dslash_num_blocks(const DslashCall &c);
 

@@ -856,12 +856,15 @@ __device__ __forceinline__ void store_nt(V *p, const V &v)
 //   CS: stored input (and xpay operand) complex, CO: stored output complex,
 //   CG: stored link complex, A: accumulation real type.
 //   SITE_MINOR: lanes enumerate sites fastest (reference SoA layout).
+// hw_blk: the workgroup's index among the launch's a.nblocks workgroups in
+// hardware dispatch order (blockIdx.x for a launch of its own; the offset
+// index inside a batched work-list launch, whose ranges start at multiples of
+// the XCD count so that hw_blk % 8 is still the XCD).
 template <int N, int R, int SL, int GL, bool LDS, bool SITE_MINOR, class CS, class CG, class CO, class A>
-__device__ __forceinline__ void dslash_body(const DslashArgs &a)
+__device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t blk =
-      a.xcd_gran ? xcd_cyclic(blockIdx.x, a.xcd_gran) : (a.xcd_map ? xcd_remap(blockIdx.x, a.nblocks) : blockIdx.x);
+  const uint32_t blk = a.xcd_gran ? xcd_cyclic(hw_blk, a.xcd_gran) : (a.xcd_map ? xcd_remap(hw_blk, a.nblocks) : hw_blk);
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;
@@ -1177,9 +1180,9 @@ __global__ void __launch_bounds__(256) dslash_kernel(const DslashArgs a)
   if constexpr (KF > 0) {
     DslashArgs b = a;
     b.k = KF;
-    dslash_body<N, R, SL, GL, LDS, SITE_MINOR, CS, CG, CO, A>(b);
+    dslash_body<N, R, SL, GL, LDS, SITE_MINOR, CS, CG, CO, A>(b, blockIdx.x);
   } else {
-    dslash_body<N, R, SL, GL, LDS, SITE_MINOR, CS, CG, CO, A>(a);
+    dslash_body<N, R, SL, GL, LDS, SITE_MINOR, CS, CG, CO, A>(a, blockIdx.x);
   }
 }
 

@@ -1,7 +1,9 @@
 """Batched hopping term (qcuMrhsBatchCreate / Run / Destroy): uneven
 problems (SU(2)..SU(5), different lattices and nrhs, FP32) applied by one
-HIP graph, checked against the FP64 oracle and against the same kernels
-launched one by one, on several replays with eager work in between."""
+HIP graph (a single work-list kernel; with an N outside 2..5 a chain of
+kernels), checked against the FP64 oracle and against the same kernels with
+the same tiles launched one by one, on several replays with eager work in
+between."""
 import ctypes
 
 import numpy as np
@@ -21,9 +23,11 @@ PROBLEMS = [((8, 8, 8, 8), 2, 8, 0, 0), ((8, 8, 8, 16), 3, 12, 1, 0), ((8, 4, 4,
             ((4, 4, 4, 8), 5, 6, 1, 1)]
 
 
-def test_batch_graph_replays(lib, dev):
+@pytest.mark.parametrize("problems", [PROBLEMS, [((4, 4, 4, 8), 6, 4, 1, 0), ((8, 4, 4, 8), 3, 10, 0, 1)]],
+                         ids=["worklist", "chain"])
+def test_batch_graph_replays(lib, dev, problems):
     items = []
-    for i, (L, N, k, parity, dagger) in enumerate(PROBLEMS):
+    for i, (L, N, k, parity, dagger) in enumerate(problems):
         g = fields.gauge_field(L, N, seed=40 + i)
         psi = fields.gaussian_spinors(L, N, k, seed=50 + i)
         geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
@@ -41,9 +45,9 @@ def test_batch_graph_replays(lib, dev):
     try:
         # The same kernels launched one by one with the batch's (tuned) tiles.
         eager = []
-        for it in items:
-            t = _lib.Tile()
-            _lib.check(lib.qcuMrhsGetTile(it["geom"], _lib.SINGLE, it["k"], ctypes.byref(t)))
+        for i, it in enumerate(items):
+            t = _lib.Tile()  # the tile the batch runs this problem with
+            _lib.check(lib.qcuMrhsBatchGetTile(handle, i, ctypes.byref(t)))
             o = torch.empty_like(it["pin"])
             _lib.check(lib.qcuMrhsDslash(o.data_ptr(), it["pin"].data_ptr(), it["gnat"].data_ptr(), _lib.SINGLE,
                                          it["k"], it["parity"], it["dagger"], it["geom"], ctypes.byref(t), stream()))

@@ -103,9 +103,9 @@ def test_config5_batch_full_size(lib, dev):
     _lib.check(lib.qcuMrhsBatchCreate(probs, len(items), ctypes.byref(handle), stream()))
     try:
         eager = []
-        for it in items:
-            t = _lib.Tile()
-            _lib.check(lib.qcuMrhsGetTile(it["geom"], _lib.SINGLE, it["k"], ctypes.byref(t)))
+        for i, it in enumerate(items):
+            t = _lib.Tile()  # the tile the batch runs this problem with
+            _lib.check(lib.qcuMrhsBatchGetTile(handle, i, ctypes.byref(t)))
             o = torch.empty_like(it["pin"])
             _lib.check(lib.qcuMrhsDslash(o.data_ptr(), it["pin"].data_ptr(), it["gnat"].data_ptr(), _lib.SINGLE,
                                          it["k"], it["parity"], it["dagger"], it["geom"], ctypes.byref(t), stream()))
