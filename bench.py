@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--cg-only", action="store_true", help="only the CG comparison (printed as one JSON line)")
     ap.add_argument("--cg-L", type=int, nargs=4, default=[32, 32, 32, 64])
     ap.add_argument("--cg-clover", type=float, default=None, help="with --cg-only: Wilson-clover with this c_sw")
-    ap.add_argument("--cg-solver", choices=["both", "fp64", "mixed"], default="both",
+    ap.add_argument("--cg-solver", choices=["both", "fp64", "mixed", "fp8"], default="both",
                     help="with --cg-only: run one solver only (profiling)")
     ap.add_argument("--profile", action="store_true", help="headline kernel only (for rocprofv3 runs)")
     ap.add_argument("--retune", action="store_true",
@@ -67,7 +67,7 @@ def parse():
                     help="only the stream-bandwidth ceiling over buffers of these sizes in MiB")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
-    ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64"],
+    ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64", "fp8"],
                     help="storage precision of the headline kernel (profiling runs)")
     ap.add_argument("--L", type=int, nargs=4, default=[24, 24, 24, 48])
     ap.add_argument("--nrhs", type=int, default=16)
@@ -89,8 +89,9 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
         _GAUGE_CACHE.clear()
         _GAUGE_CACHE[key] = torch.view_as_real(fields.gauge_field_device(L, N, seed=seed, device=dev)).reshape(-1)
     gu = _GAUGE_CACHE[key]
-    dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16}[prec]
-    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, prec) // _lib.BYTES[prec] * 2, dtype=dt, device=dev)
+    # FP8 spinors go with binary16 links (gauge bytes and packing as HALF).
+    dt = {0: torch.float16, 1: torch.float32, 2: torch.float64, 4: torch.bfloat16, 5: torch.float16}[prec]
+    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, prec) // dt.itemsize, dtype=dt, device=dev)
     _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), prec, gu.data_ptr(), _lib.DOUBLE, geom, sp))
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed + 1)
@@ -98,7 +99,13 @@ def time_dslash(lib, _lib, torch, geom, prec, k, steps, warmup, L, N, seed=0, ti
     # parity i & 1 to the same input, so every timed launch sees the same
     # finite data (feeding outputs back in would grow the field ~4x per
     # application) and the fields touched are the two a ping-pong would touch.
-    a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32).to(dt)
+    a = torch.randn(S * 4 * N * k * 2, generator=gen, device=dev, dtype=torch.float32)
+    if prec == _lib.FP8:
+        a8 = torch.empty(S * 4 * N * k * 2, dtype=torch.uint8, device=dev)
+        _lib.check(lib.qcuMrhsConvert(a8.data_ptr(), _lib.FP8, a.data_ptr(), _lib.SINGLE, S * 4 * N * k, sp))
+        a = a8
+    else:
+        a = a.to(dt)
     ins = [a, a]
     o = torch.empty_like(a)
     outs = [o, o]
@@ -226,7 +233,7 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
     out = {"workload": f"su{N}_eo_{op}_{'x'.join(map(str, L))}_nrhs{k}_kappa{kappa}", "tol": tol,
            "sources": f"{npt} point sources + {k - npt} Gaussian"}
     sols = {}
-    for name, sloppy in (("fp64", _lib.DOUBLE), ("mixed_fp16", _lib.HALF)):
+    for name, sloppy in (("fp64", _lib.DOUBLE), ("mixed_fp16", _lib.HALF), ("mixed_fp8", _lib.FP8)):
         if name not in solvers:
             continue
         x = torch.empty_like(b)
@@ -259,6 +266,10 @@ def cg_benchmark(lib, _lib, torch, L=(32, 32, 32, 64), N=3, k=16, kappa=0.1350, 
     n = sols["fp64"].permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
     out["max_rel_solution_diff"] = float((d / n).max())
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_fp16"]["seconds"]
+    if "mixed_fp8" in sols:
+        d8 = (sols["mixed_fp8"] - sols["fp64"]).permute(3, 0, 1, 2, 4).reshape(k, -1).norm(dim=1)
+        out["mixed_fp8"]["max_rel_solution_diff"] = float((d8 / n).max())
+        out["mixed_fp8"]["speedup_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_fp8"]["seconds"]
     return out
 
 
@@ -575,16 +586,18 @@ def main():
         print(json.dumps(clover_benchmark(lib, _lib, torch)), flush=True)
         return
     if args.cg_only:
-        solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",)}[args.cg_solver]
+        solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",),
+                   "fp8": ("fp64", "mixed_fp16", "mixed_fp8")}[args.cg_solver]
         print(json.dumps(cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), solvers=solvers, csw=args.cg_clover)),
               flush=True)
         return
-    pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE}[args.precision]
-    pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8}[args.precision]
+    pcode = {"fp16": _lib.HALF, "bf16": _lib.BF16, "fp32": _lib.SINGLE, "fp64": _lib.DOUBLE,
+             "fp8": _lib.FP8}[args.precision]
+    pb = {"fp16": 2, "bf16": 2, "fp32": 4, "fp64": 8, "fp8": 1}[args.precision]
     tile_arg = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
     ms, tile = time_dslash(lib, _lib, torch, geom, pcode, k, args.steps, args.warmup, L, N, tile=tile_arg)
     value = S * k / (ms * 1e-3)
-    ba = bytes_alg(S, k, N, pb, pb, pb)
+    ba = bytes_alg(S, k, N, pb, pb, 2 if pcode == _lib.FP8 else pb)  # FP8 spinors use binary16 links
     achieved = ba / (ms * 1e-3) / 1e9
     wl = f"su3_mrhs_eo_wilson_dslash_{'x'.join(map(str, L))}_nrhs{k}_{args.precision}"
     traffic, traffic_tile = traffic_from_profile(f"{wl}_traffic.json")
@@ -602,7 +615,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp16": "f16 storage, f32 arithmetic", "bf16": "bf16 storage, f32 arithmetic", "fp32": "f32",
-                  "fp64": "f64"}[args.precision],
+                  "fp64": "f64", "fp8": "e4m3 spinor storage, f16 links, f32 arithmetic"}[args.precision],
         "data": "synthetic (seeded SU(3) links, Gaussian spinors)",
         "config": {"workload": wl, "lattice": list(L), "nColors": N, "nrhs": k, "storage": args.precision,
                    "tile": dict(zip(("rhs_per_lane", "block", "link_lds", "t_block", "z_block", "cache_hints",
@@ -615,9 +628,10 @@ def main():
     }
     if not args.profile and not args.no_extra:
         extra = {}
-        for name, prec, b in (("fp16", _lib.HALF, 2), ("bf16", _lib.BF16, 2), ("fp64", _lib.DOUBLE, 8)):
+        for name, prec, b in (("fp16", _lib.HALF, 2), ("bf16", _lib.BF16, 2), ("fp64", _lib.DOUBLE, 8),
+                              ("fp8_spinors_fp16_links", _lib.FP8, 1)):
             ms2, t2 = time_dslash(lib, _lib, torch, geom, prec, k, max(20, args.steps // 4), 10, L, N)
-            bb = bytes_alg(S, k, N, b, b, b)
+            bb = bytes_alg(S, k, N, b, b, 2 if prec == _lib.FP8 else b)
             extra[name] = {"ms_per_step": ms2, "sites_rhs_per_s": S * k / (ms2 * 1e-3),
                            "hbm_frac": bb / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS, "tile": t2}
         for name, prec, b, reals in (("two_row_links_fp32", _lib.SINGLE, 4, 12), ("two_row_links_fp16", _lib.HALF, 2, 12),
@@ -644,7 +658,8 @@ def main():
         extra["qcu_h_end_to_end"] = qcu_h_end_to_end(lib, _lib, torch)
         line["extra"] = extra
         if not args.no_cg:
-            line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L))
+            line["cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L),
+                                      solvers=("fp64", "mixed_fp16", "mixed_fp8"))
             line["clover_cg"] = cg_benchmark(lib, _lib, torch, L=tuple(args.cg_L), repeats=1, csw=1.0)
     if not args.no_cpu and not args.profile:
         line["cpu_baseline"] = cpu_baseline(L, N, k)

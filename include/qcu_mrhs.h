@@ -12,7 +12,10 @@
  *  - Return value 0 = success; nonzero = error, message via qcuMrhsLastError().
  *  - Precision codes: QCU_MRHS_HALF (IEEE binary16 storage, FP32 arithmetic),
  *    QCU_MRHS_SINGLE, QCU_MRHS_DOUBLE, QCU_MRHS_BF16 (bfloat16 storage, FP32
- *    arithmetic). They extend QCU_PRECISION of qcu_enum.h.
+ *    arithmetic), QCU_MRHS_FP8 (OCP FP8 E4M3 spinors, 2 bytes per complex,
+ *    saturating at +-448, with binary16 links: the gauge of an FP8 operator is
+ *    packed and sized as QCU_MRHS_HALF; FP32 arithmetic; nColors = 3). They
+ *    extend QCU_PRECISION of qcu_enum.h.
  *  - Layouts (complex numbers interleaved re, im; S = Lx*Ly*Lz*Lt/2 sites of
  *    one parity, checkerboard index idx = ((t*Lz+z)*Ly+y)*(Lx/2)+x/2):
  *      user spinor   [S][4][N]            (one buffer per RHS; the qcu.h layout)
@@ -33,7 +36,7 @@
 extern "C" {
 #endif
 
-enum { QCU_MRHS_HALF = 0, QCU_MRHS_SINGLE = 1, QCU_MRHS_DOUBLE = 2, QCU_MRHS_BF16 = 4 };
+enum { QCU_MRHS_HALF = 0, QCU_MRHS_SINGLE = 1, QCU_MRHS_DOUBLE = 2, QCU_MRHS_BF16 = 4, QCU_MRHS_FP8 = 5 };
 
 typedef struct qcuMrhsGeom_s {
   int L[4];    /* local lattice Lx, Ly, Lz, Lt (all even) */
@@ -233,10 +236,10 @@ typedef struct qcuMrhsCgParam_s {
   double tol;            /* target ||M^+b - M^+M^ x|| / ||M^+b|| per RHS */
   int max_iter;          /* iteration cap */
   int reliable_interval; /* mixed precision: iterations between FP64 residual updates */
-  int sloppy_prec;       /* QCU_MRHS_DOUBLE: FP64 CG; HALF / BF16 / SINGLE: mixed precision */
+  int sloppy_prec;       /* QCU_MRHS_DOUBLE: FP64 CG; HALF / BF16 / SINGLE / FP8: mixed precision */
   double reliable_delta; /* mixed precision: true residual recomputed once the iterated residual of a
                             RHS fell by this factor since its last update (0 = precision default:
-                            HALF 3e-3, SINGLE 1e-3, BF16 1e-1) */
+                            HALF 3e-3, SINGLE 1e-3, BF16 1e-1, FP8 3e-1) */
   /* outputs */
   int iterations;
   int reliable_updates;

@@ -31,6 +31,7 @@ int bytes_of(int prec)
     case PREC_SINGLE: return 8;
     case PREC_HALF: return 4;
     case PREC_BF16: return 4;
+    case PREC_FP8: return 2;
     default: return 0;
   }
 }
@@ -229,7 +230,7 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
     // and fit, for 16-bit input (FP32 accumulation): at 32^3x64, 16 RHS,
     // the scaled FP16 operator takes 624 instead of 644 us; for FP32 input
     // it made no difference (1,094 / 1,097 us).
-    const bool half_in = c.prec_s == PREC_HALF || c.prec_s == PREC_BF16;
+    const bool half_in = c.prec_s == PREC_HALF || c.prec_s == PREC_BF16 || c.prec_s == PREC_FP8;
     const size_t efac = size_t(4) * c.k * sizeof(float);
     if (half_in && !a.plain_out && shm + efac <= size_t(kMaxDynLds)) {
       a.efac_off = uint32_t(shm);

@@ -61,6 +61,7 @@ int prec_bytes(int prec)
     case PREC_SINGLE: return 8;
     case PREC_HALF: return 4;
     case PREC_BF16: return 4;
+    case PREC_FP8: return 2;
     default: return 0;
   }
 }
@@ -179,7 +180,7 @@ Tile tile_for(const Geom &g, int prec, int k)
   }
   Tile t = default_tile(g, k, prec);
   // Fall back to a plain tile when the preferred one is not instantiated.
-  KernelKey key{SL_PACKED, GL_NATIVE, g.N, t.R, t.lds, 0, prec, prec, prec};
+  KernelKey key{SL_PACKED, GL_NATIVE, g.N, t.R, t.lds, 0, prec, gauge_prec_for(prec), prec};
   if (k % t.R || !find_kernel(key)) {
     t.R = 1;
     t.lds = 0;
@@ -187,7 +188,7 @@ Tile tile_for(const Geom &g, int prec, int k)
   // LDS staging only within the per-workgroup budget.
   if (t.lds) {
     const int P = k / t.R;
-    const size_t shm = size_t((t.block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(prec);
+    const size_t shm = size_t((t.block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
     if (shm > size_t(kMaxDynLds)) t.lds = 0;
   }
   return t;
@@ -212,12 +213,12 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
   for (int R : {1, 2, 4}) {
     if (k % R) continue;
     for (int lds : {0, 1}) {
-      KernelKey key{SL_PACKED, GL_NATIVE, g.N, R, lds, 0, prec, prec, prec};
+      KernelKey key{SL_PACKED, GL_NATIVE, g.N, R, lds, 0, prec, gauge_prec_for(prec), prec};
       if (!find_kernel(key)) continue;
       for (int block : {128, 256}) {
         if (lds) {
           const int P = k / R;
-          const size_t shm = size_t((block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(prec);
+          const size_t shm = size_t((block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
           if (shm > size_t(kMaxDynLds)) continue;
           if (P < 2) continue;  // no link reuse inside a workgroup
         }
@@ -299,7 +300,8 @@ bool dslash_ex_call(DslashCall *cp, void *out, int out_prec, const void *in, con
   DslashCall &c = *cp;
   c.sl = SL_PACKED;
   c.gl = GL_NATIVE;
-  c.prec_s = c.prec_g = in_prec;
+  c.prec_s = in_prec;
+  c.prec_g = gauge_prec_for(in_prec);
   c.prec_o = out_prec;
   c.geom = g;
   c.tile = tile_for(g, in_prec, nrhs);
@@ -347,7 +349,8 @@ int run_packed_dslash(void *out, const void *in, const void *x, double xa, doubl
   DslashCall c;
   c.sl = SL_PACKED;
   c.gl = GL_NATIVE;
-  c.prec_s = c.prec_g = c.prec_o = prec;
+  c.prec_s = c.prec_o = prec;
+  c.prec_g = gauge_prec_for(prec);
   c.geom = g;
   c.tile = tile ? *tile : tile_for(g, prec, k);
   c.k = k;
@@ -386,7 +389,8 @@ size_t qcuMrhsGaugeBytes(const qcuMrhsGeom *gc, int prec)
   Geom g;
   std::string err;
   if (!geom_from_c(gc, &g, &err) || !prec_bytes(prec)) return 0;
-  return size_t(sites_per_parity(g)) * 2 * 8 * g.N * g.N * prec_bytes(prec);
+  // FP8 spinors go with binary16 links: the same bytes and layout as HALF.
+  return size_t(sites_per_parity(g)) * 2 * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
 }
 
 int qcuMrhsPackGauge(void *dst, int dst_prec, const void *src, int src_prec, const qcuMrhsGeom *gc, void *stream)
@@ -395,7 +399,8 @@ int qcuMrhsPackGauge(void *dst, int dst_prec, const void *src, int src_prec, con
   std::string err;
   if (!geom_from_c(gc, &g, &err)) return api_fail(err);
   if (!dst || !src) return api_fail("null gauge pointer");
-  if (pack_gauge(dst, dst_prec, src, src_prec, g.N, g.L, static_cast<hipStream_t>(stream), &err)) return api_fail(err);
+  if (pack_gauge(dst, gauge_prec_for(dst_prec), src, src_prec, g.N, g.L, static_cast<hipStream_t>(stream), &err))
+    return api_fail(err);
   return 0;
 }
 
@@ -530,7 +535,8 @@ int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const v
   DslashCall c;
   c.sl = SL_AOS_PTRS;
   c.gl = GL_NATIVE;
-  c.prec_s = c.prec_g = c.prec_o = prec;
+  c.prec_s = c.prec_o = prec;
+  c.prec_g = gauge_prec_for(prec);
   c.geom = g;
   c.k = nrhs;
   c.parity = parity;

@@ -11,7 +11,11 @@ namespace qmrhs {
 
 // ---------------------------------------------------------------------------
 // Storage precisions (values of QCU_PRECISION plus bf16).
-enum Prec : int { PREC_HALF = 0, PREC_SINGLE = 1, PREC_DOUBLE = 2, PREC_BF16 = 4 };
+enum Prec : int { PREC_HALF = 0, PREC_SINGLE = 1, PREC_DOUBLE = 2, PREC_BF16 = 4, PREC_FP8 = 5 };
+
+// Storage precision of the links that go with spinors of precision p: FP8
+// spinors use binary16 links (three mantissa bits would be the whole link).
+inline constexpr int gauge_prec_for(int p) { return p == PREC_FP8 ? int(PREC_HALF) : p; }
 
 // One complex number as stored in HBM.
 struct alignas(4) c16 {  // IEEE binary16 pair
@@ -20,6 +24,10 @@ struct alignas(4) c16 {  // IEEE binary16 pair
 struct alignas(4) cb16 {  // bfloat16 pair, raw bits
   uint16_t x, y;
 };
+struct alignas(2) c8 {  // OCP FP8 E4M3 pair (4 exponent bits, bias 7, 3 mantissa bits, max 448), raw bits
+  uint8_t x, y;
+};
+constexpr float kFp8Max = 448.0f;
 using c32 = float2;
 using c64 = double2;
 
@@ -29,6 +37,7 @@ template <> struct StoreInfo<c64> { static constexpr int bytes = 16; static cons
 template <> struct StoreInfo<c32> { static constexpr int bytes = 8; static constexpr int prec = PREC_SINGLE; };
 template <> struct StoreInfo<c16> { static constexpr int bytes = 4; static constexpr int prec = PREC_HALF; };
 template <> struct StoreInfo<cb16> { static constexpr int bytes = 4; static constexpr int prec = PREC_BF16; };
+template <> struct StoreInfo<c8> { static constexpr int bytes = 2; static constexpr int prec = PREC_FP8; };
 
 // ---------------------------------------------------------------------------
 // Accumulation complex type.
@@ -74,6 +83,13 @@ template <class A> __device__ __forceinline__ cpx<A> ld_cvt(const cb16 &v)
 {
   return cpx<A>{A(bf16_to_f32(v.x)), A(bf16_to_f32(v.y))};
 }
+// FP8 E4M3 pair -> two floats with one gfx950 conversion (exact).
+template <class A> __device__ __forceinline__ cpx<A> ld_cvt(const c8 &v)
+{
+  const int w = int(uint32_t(v.x) | (uint32_t(v.y) << 8));
+  const auto f = __builtin_amdgcn_cvt_pk_f32_fp8(w, false);
+  return cpx<A>{A(f[0]), A(f[1])};
+}
 
 // Convert an accumulated complex to a storage complex.
 template <class C> struct StCvt;
@@ -89,6 +105,20 @@ template <> struct StCvt<c16> {
     c16 r;
     r.x = _Float16(float(z.re));
     r.y = _Float16(float(z.im));
+    return r;
+  }
+};
+template <> struct StCvt<c8> {
+  // Round to nearest even, saturating at +-448 (the format has no infinity;
+  // a value the hardware would turn into NaN is clamped first).
+  template <class A> __device__ __forceinline__ static c8 go(cpx<A> z)
+  {
+    const float re = __builtin_fminf(__builtin_fmaxf(float(z.re), -kFp8Max), kFp8Max);
+    const float im = __builtin_fminf(__builtin_fmaxf(float(z.im), -kFp8Max), kFp8Max);
+    const uint32_t w = uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false));
+    c8 r;
+    r.x = uint8_t(w & 0xffu);
+    r.y = uint8_t((w >> 8) & 0xffu);
     return r;
   }
 };

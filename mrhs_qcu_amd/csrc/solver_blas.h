@@ -14,6 +14,7 @@ struct CgState {
   int *done, *sloppy_conv;
   float *sc_p, *sc_t, *sc_q, *sc_t2;  // sloppy storage scales (true = stored * sc)
   float *sc_p_old;                    // scale of the stored p before cg_beta replaced sc_p
+  float scale_div = 1024.0f;          // sc_p = (bound of |p|) / scale_div; t, q, t2 16x, 4x, 64x that
 };
 
 // Workgroups of every vector kernel for n complex numbers and k RHS (the

@@ -19,6 +19,7 @@ template <> struct Acc<c64> { using T = double; };
 template <> struct Acc<c32> { using T = float; };
 template <> struct Acc<c16> { using T = float; };
 template <> struct Acc<cb16> { using T = float; };
+template <> struct Acc<c8> { using T = float; };
 
 __device__ __forceinline__ void block_partial(double v, int rhs, int k, double *partial)
 {
@@ -233,7 +234,7 @@ __global__ void beta_kernel(CgState s, int k, double stop2)
   if (live) s.rr[r] = rn;
   s.sloppy_conv[r] = (rn <= stop2 * s.b2[r]) ? 1 : 0;
   const double bound = sqrt(rn) + fabs(beta) * s.pnorm[r];
-  float sp = float(bound / 1024.0);
+  float sp = float(bound / double(s.scale_div));
   if (!(sp > 1e-30f)) sp = 1e-30f;
   s.sc_p_old[r] = s.sc_p[r];
   s.sc_p[r] = sp;
@@ -256,7 +257,7 @@ __global__ void init_state_kernel(CgState s, int k)
   s.pnorm2[r] = 0.0;
   s.done[r] = b2 > 0.0 ? 0 : 1;
   s.sloppy_conv[r] = s.done[r];
-  float sp = float(sqrt(b2) / 1024.0);
+  float sp = float(sqrt(b2) / double(s.scale_div));
   if (!(sp > 1e-30f)) sp = 1e-30f;
   s.sc_p[r] = sp;
   s.sc_p_old[r] = sp;
@@ -289,7 +290,7 @@ __global__ void orth_kernel(CgState s, int k)
   const double gamma = (!s.done[r] && rr > 0.0) ? 1.0 - s.beta[r] * s.rdotp[r] / rr : 1.0;
   s.gamma[r] = gamma;
   const double bound = fabs(gamma) * sqrt(rr) + fabs(s.beta[r]) * s.pnorm[r];
-  float sp = float(bound / 1024.0);
+  float sp = float(bound / double(s.scale_div));
   if (!(sp > 1e-30f)) sp = 1e-30f;
   s.sc_p[r] = sp;
   s.sc_t[r] = 16.0f * sp;
@@ -330,6 +331,7 @@ hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *
     switch (prec_s) {
       case PREC_HALF: QM_UP(c32, c16); break;
       case PREC_BF16: QM_UP(c32, cb16); break;
+      case PREC_FP8: QM_UP(c32, c8); break;
       default: QM_UP(c32, c32);
     }
   }
@@ -348,6 +350,9 @@ hipError_t blas_norm2(int prec, const void *v, double *partial, long long n, int
       break;
     case PREC_HALF:
       hipLaunchKernelGGL(norm2_kernel<c16>, QM_GRID(n, k), static_cast<const c16 *>(v), partial, n, k);
+      break;
+    case PREC_FP8:
+      hipLaunchKernelGGL(norm2_kernel<c8>, QM_GRID(n, k), static_cast<const c8 *>(v), partial, n, k);
       break;
     default:
       hipLaunchKernelGGL(norm2_kernel<cb16>, QM_GRID(n, k), static_cast<const cb16 *>(v), partial, n, k);
@@ -368,6 +373,7 @@ hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int
     case PREC_DOUBLE: QM_RES(CK, CA, c64); break; \
     case PREC_SINGLE: QM_RES(CK, CA, c32); break; \
     case PREC_HALF: QM_RES(CK, CA, c16); break;   \
+    case PREC_FP8: QM_RES(CK, CA, c8); break;     \
     default: QM_RES(CK, CA, cb16);                \
   }
   if (prec_k == PREC_DOUBLE) {
@@ -393,6 +399,7 @@ hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, int pre
     case PREC_DOUBLE: QM_ACC(CK, c64); break;   \
     case PREC_SINGLE: QM_ACC(CK, c32); break;   \
     case PREC_HALF: QM_ACC(CK, c16); break;     \
+    case PREC_FP8: QM_ACC(CK, c8); break;       \
     default: QM_ACC(CK, cb16);                  \
   }
   if (prec_k == PREC_DOUBLE) {
@@ -411,6 +418,7 @@ hipError_t blas_zero(int prec, void *v, long long n, int k, hipStream_t st)
     case PREC_DOUBLE: hipLaunchKernelGGL(zero_kernel<c64>, QM_GRID(n, k), static_cast<c64 *>(v), n); break;
     case PREC_SINGLE: hipLaunchKernelGGL(zero_kernel<c32>, QM_GRID(n, k), static_cast<c32 *>(v), n); break;
     case PREC_HALF: hipLaunchKernelGGL(zero_kernel<c16>, QM_GRID(n, k), static_cast<c16 *>(v), n); break;
+    case PREC_FP8: hipLaunchKernelGGL(zero_kernel<c8>, QM_GRID(n, k), static_cast<c8 *>(v), n); break;
     default: hipLaunchKernelGGL(zero_kernel<cb16>, QM_GRID(n, k), static_cast<cb16 *>(v), n);
   }
   return hipGetLastError();

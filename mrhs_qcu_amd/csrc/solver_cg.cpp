@@ -125,7 +125,7 @@ struct Solver {
     c.sl = SL_PACKED;
     c.gl = GL_NATIVE;
     c.prec_s = prec_in;
-    c.prec_g = prec_in;
+    c.prec_g = gauge_prec_for(prec_in);
     c.prec_o = prec_out;
     c.geom = g;
     c.tile = tile_for(g, prec_in, k);
@@ -238,8 +238,10 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   const auto t0 = std::chrono::steady_clock::now();
   if (k < 1 || k > kBlasMaxRhs) return *err = "nrhs must be in 1..256", 1;
   const int ps = prm->sloppy_prec;
-  if (ps != PREC_DOUBLE && ps != PREC_SINGLE && ps != PREC_HALF && ps != PREC_BF16)
+  if (ps != PREC_DOUBLE && ps != PREC_SINGLE && ps != PREC_HALF && ps != PREC_BF16 && ps != PREC_FP8)
     return *err = "unsupported sloppy precision", 1;
+  if (ps == PREC_FP8 && g.N != 3) return *err = "the FP8 sloppy operator is instantiated for nColors = 3", 1;
+  if (ps == PREC_FP8 && clover64) return *err = "the clover solve has no FP8 sloppy precision", 1;
   if (!(prm->tol > 0.0)) return *err = "tolerance must be positive", 1;
   const bool clov = clover64 != nullptr;
   if (clov && (!clover_inv64 || g.N > 5)) return *err = "the clover solve needs A and A^-1 and nColors <= 5", 1;
@@ -275,7 +277,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   void *t_s = mixed ? mem.bytes(bs, err, &ok) : t64;
   void *q_s = mixed ? mem.bytes(bs, err, &ok) : ax64;
   void *t2_s = mixed ? mem.bytes(bs, err, &ok) : t64;
-  void *gs = mixed ? mem.bytes(size_t(s.S) * 2 * 8 * g.N * g.N * prec_bytes(ps), err, &ok) : const_cast<void *>(gauge64);
+  const int pg = gauge_prec_for(ps);  // sloppy links (FP8 spinors: binary16 links)
+  void *gs = mixed ? mem.bytes(size_t(s.S) * 2 * 8 * g.N * g.N * prec_bytes(pg), err, &ok) : const_cast<void *>(gauge64);
   const long long nb_blas = blas_num_blocks(n, k);
   // Partial sums for the largest workgroup count any tile can use
   // (one RHS per lane, 64-lane workgroups).
@@ -302,6 +305,9 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   cs.sc_q = mem.alloc<float>(k, err, &ok);
   cs.sc_t2 = mem.alloc<float>(k, err, &ok);
   cs.sc_p_old = mem.alloc<float>(k, err, &ok);
+  // Stored sloppy fields are sized from a bound of their 2-norm: at most
+  // 1024 (binary16, max 65504) or 256 (FP8 E4M3, max 448) per RHS for p.
+  cs.scale_div = ps == PREC_FP8 ? 256.0f : 1024.0f;
   double *rr_true = mem.alloc<double>(k, err, &ok);
   // Clover: FP64 scratch for M^ in FP64 (also the loop's fields when the
   // solve is all FP64), sloppy copies of A and A^-1 and two sloppy fields.
@@ -333,7 +339,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   if (!ok) return 1;
   if (mixed) {
     std::string e2;
-    if (convert_complex(gs, ps, gauge64, PREC_DOUBLE, s.S * 2 * 8 * g.N * g.N, st, &e2)) return *err = e2, 1;
+    if (convert_complex(gs, pg, gauge64, PREC_DOUBLE, s.S * 2 * 8 * g.N * g.N, st, &e2)) return *err = e2, 1;
   }
   // Tune the operator tiles for this shape once per process (the tuner
   // overwrites the scratch fields it times on; they are initialised below).
@@ -359,7 +365,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     }
     if (!tuned(g, PREC_SINGLE, k) && !tune_now(t32, x32, g32, PREC_SINGLE, k, g, st, err)) return 1;
   }
-  const bool scaled = mixed && ps == PREC_HALF;
+  const bool scaled = mixed && (ps == PREC_HALF || ps == PREC_FP8);
   const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
   const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
   const float *sc_p_old = scaled ? cs.sc_p_old : nullptr;
@@ -402,7 +408,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   // residual then cannot drift apart), or every live RHS reached the
   // tolerance, or reliable_interval iterations passed.
   const double delta = prm->reliable_delta > 0.0 ? prm->reliable_delta
-                       : (ps == PREC_SINGLE ? 1e-3 : (ps == PREC_HALF ? 3e-3 : 1e-1));
+                       : (ps == PREC_SINGLE ? 1e-3 : (ps == PREC_HALF ? 3e-3 : (ps == PREC_FP8 ? 3e-1 : 1e-1)));
   const double delta2 = mixed ? delta * delta : 0.0;
   // QCU_MRHS_CG_VERBOSE=1 prints the iterated and true residuals.
   const char *verb_env = getenv("QCU_MRHS_CG_VERBOSE");

@@ -170,6 +170,7 @@ static hipError_t pack_dispatch_src(bool unpack, int src_prec, void *packed, voi
     case PREC_SINGLE: return launch_pack<CD, c32>(unpack, packed, ptrs, k, M, S, st);
     case PREC_HALF: return launch_pack<CD, c16>(unpack, packed, ptrs, k, M, S, st);
     case PREC_BF16: return launch_pack<CD, cb16>(unpack, packed, ptrs, k, M, S, st);
+    case PREC_FP8: return launch_pack<CD, c8>(unpack, packed, ptrs, k, M, S, st);
     default: *ok = false; return hipSuccess;
   }
 }
@@ -186,6 +187,7 @@ int pack_spinors(void *dst, int dst_prec, const void *const *src, int src_prec, 
     case PREC_SINGLE: e = pack_dispatch_src<c32>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
     case PREC_HALF: e = pack_dispatch_src<c16>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
     case PREC_BF16: e = pack_dispatch_src<cb16>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
+    case PREC_FP8: e = pack_dispatch_src<c8>(false, src_prec, dst, ptrs, k, M, S, st, &ok); break;
     default: ok = false;
   }
   if (!ok) {
@@ -211,6 +213,7 @@ int unpack_spinors(void *const *dst, int dst_prec, const void *src, int src_prec
     case PREC_SINGLE: e = pack_dispatch_src<c32>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
     case PREC_HALF: e = pack_dispatch_src<c16>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
     case PREC_BF16: e = pack_dispatch_src<cb16>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
+    case PREC_FP8: e = pack_dispatch_src<c8>(true, src_prec, packed, dst, k, M, S, st, &ok); break;
     default: ok = false;
   }
   if (!ok) {
@@ -481,6 +484,10 @@ static hipError_t convert_src(int src_prec, void *dst, const void *src, long lon
       hipLaunchKernelGGL((convert_kernel<CD, cb16>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
                          static_cast<const cb16 *>(src), n);
       break;
+    case PREC_FP8:
+      hipLaunchKernelGGL((convert_kernel<CD, c8>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(dst),
+                         static_cast<const c8 *>(src), n);
+      break;
     default: *ok = false; return hipSuccess;
   }
   return hipGetLastError();
@@ -497,6 +504,7 @@ int convert_complex(void *dst, int dst_prec, const void *src, int src_prec, long
     case PREC_SINGLE: e = convert_src<c32>(src_prec, dst, src, n, st, &ok); break;
     case PREC_HALF: e = convert_src<c16>(src_prec, dst, src, n, st, &ok); break;
     case PREC_BF16: e = convert_src<cb16>(src_prec, dst, src, n, st, &ok); break;
+    case PREC_FP8: e = convert_src<c8>(src_prec, dst, src, n, st, &ok); break;
     default: ok = false;
   }
   if (!ok) {

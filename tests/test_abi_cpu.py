@@ -278,7 +278,7 @@ def test_pinned_tile_table():
     """The tile table shipped beside the library: every shape line has the
     17 fields of the tune-cache format, a launchable tile (R divides nrhs,
     block a multiple of 64 up to 256) and lists the benchmark and solver
-    shapes (24^3x48 and 32^3x64, 16 RHS, SU(3), every precision)."""
+    shapes (24^3x48 and 32^3x64, 16 RHS, SU(3), every precision, FP8 included)."""
     path = os.path.join(ROOT, "mrhs_qcu_amd", "tiles_gfx950.txt")
     shapes = set()
     for line in open(path):
@@ -291,5 +291,5 @@ def test_pinned_tile_table():
         assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
         shapes.add((N, k, prec, (Lx, Ly, Lz, Lt), mask))
     for L in ((24, 24, 24, 48), (32, 32, 32, 64)):
-        for prec in (0, 1, 2, 4):
+        for prec in (0, 1, 2, 4, 5):
             assert (3, 16, prec, L, 0) in shapes, (L, prec)
