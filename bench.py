@@ -65,6 +65,8 @@ def parse():
                     help="only the split-lattice face exchange and clover timings")
     ap.add_argument("--stream", type=float, nargs="*", default=None,
                     help="only the stream-bandwidth ceiling over buffers of these sizes in MiB")
+    ap.add_argument("--link-block", action="store_true",
+                    help="only the VALU vs MFMA link x half-spinor block benchmark (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of the headline field size (counter calibration)")
     ap.add_argument("--precision", type=str, default="fp32", choices=["fp16", "bf16", "fp32", "fp64", "fp8"],
@@ -465,8 +467,40 @@ def qcu_h_end_to_end(lib, _lib, torch, L=(24, 24, 24, 48), N=3, k=16, reps=20):
         lib.startDslash(i & 1, 0)
     wall = (time.perf_counter() - t0) / reps
     del gu
+    # What a push could do with one input: copy it whole (what it does), or
+    # write it into its slot of a 16-RHS packed field (8 bytes per 128-byte
+    # line), the alternative to a copy plus one full pack at startDslash.
+    nbytes = S * 4 * N * 8
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    src = ins[0]
+    dst = torch.empty(nbytes * 16 // 4, dtype=torch.float32, device=dev)
+    per_rhs = {}
+    for mode, name in ((2, "contiguous_copy_us"), (4, "into_16_rhs_packed_slot_us")):
+        gbs = ctypes.c_double()
+        _lib.check(lib.qcuMrhsStreamBandwidth(mode, dst.data_ptr(), src.data_ptr(), nbytes, 20, ctypes.byref(gbs), sp))
+        per_rhs[name] = 2 * nbytes / (gbs.value * 1e9) * 1e6
     return {"workload": f"qcu.h push x{k} + startDslash, su{N} {'x'.join(map(str, L))} fp32 user layout",
-            "ms_per_call": wall * 1e3, "sites_rhs_per_s": S * k / wall}
+            "ms_per_call": wall * 1e3, "sites_rhs_per_s": S * k / wall, "one_rhs_device_time": per_rhs}
+
+
+def link_block_benchmark(lib, _lib, torch, cases=((3, 16), (4, 24), (5, 6)), reps=2000):
+    """The hop's link x half-spinor block products, Y (N x 2k) = U H, on the
+    vector ALUs (the production kernels' form) and on the matrix cores
+    (v_mfma_f32_16x16x4_f32 on the real embedding), operands reloaded from
+    cache per block or held in registers: useful FP32 TFLOP/s of each and
+    their errors against an FP64 product."""
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for N, k in cases:
+        row = {}
+        for resident, name in ((0, "operands_from_cache"), (1, "operands_in_registers")):
+            tf = (ctypes.c_double * 2)()
+            err = (ctypes.c_double * 2)()
+            _lib.check(lib.qcuMrhsLinkBlockBench(N, k, reps, resident, tf, err, sp))
+            row[name] = {"valu_tflops": tf[0], "mfma_tflops": tf[1]}
+            row["valu_rel_err"], row["mfma_rel_err"] = err[0], err[1]
+        out[f"su{N}_nrhs{k}"] = row
+    return out
 
 
 def cpu_model():
@@ -585,6 +619,9 @@ def main():
         print(json.dumps(faces_benchmark(lib, _lib, torch)), flush=True)
         print(json.dumps(clover_benchmark(lib, _lib, torch)), flush=True)
         return
+    if args.link_block:
+        print(json.dumps(link_block_benchmark(lib, _lib, torch)), flush=True)
+        return
     if args.cg_only:
         solvers = {"both": ("fp64", "mixed_fp16"), "fp64": ("fp64",), "mixed": ("mixed_fp16",),
                    "fp8": ("fp64", "mixed_fp16", "mixed_fp8")}[args.cg_solver]
@@ -653,6 +690,7 @@ def main():
             "tile": t3}
         extra["stream_ceiling_GBs"] = stream_benchmark(lib, _lib, torch, sizes_mb=(510, 2048))
         extra["batch"] = batch_benchmark(lib, _lib, torch)
+        extra["link_block_valu_vs_mfma"] = link_block_benchmark(lib, _lib, torch)
         extra["faces"] = faces_benchmark(lib, _lib, torch)
         extra["clover"] = clover_benchmark(lib, _lib, torch)
         extra["qcu_h_end_to_end"] = qcu_h_end_to_end(lib, _lib, torch)

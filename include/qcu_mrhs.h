@@ -225,10 +225,26 @@ int qcuMrhsKernelCount(void);
 /* ---- measurement -------------------------------------------------------- */
 /* Streaming bandwidth of the GPU, the ceiling the dslash's roofline fraction
  * is read against: mode 0 reads `bytes` of src, 1 writes `bytes` of dst,
- * 2 copies src to dst, 3 reads and rewrites dst in place; 16-byte accesses
- * over every CU, one warm-up pass, then `reps` timed passes. *gbs receives the
- * bytes moved (read + written) per second in GB/s. Synchronous. */
+ * 2 copies src to dst, 3 reads and rewrites dst in place, 4 copies the 8-byte
+ * elements of src to dst at a 128-byte stride (one FP32 right-hand side into
+ * its slot of a 16-RHS packed field; dst must hold 16 x bytes); 16-byte
+ * accesses over every CU, one warm-up pass, then `reps` timed passes. *gbs
+ * receives the useful bytes moved (read + written) per second in GB/s.
+ * Synchronous. */
 int qcuMrhsStreamBandwidth(int mode, void *dst, const void *src, size_t bytes, int reps, double *gbs, void *stream);
+
+/* Link x half-spinor block products of the SU(N) hop, Y (N x 2k) = U (N x N)
+ * H (N x 2k) complex, timed on the vector ALUs (the production kernels'
+ * form) and on the matrix cores (v_mfma_f32_16x16x4_f32 on the real
+ * embedding [[Ur, -Ui], [Ui, Ur]]); operands reloaded per block from
+ * cache-resident tables (resident = 0: products plus operand loads) or held in
+ * registers (resident = 1: products alone). No counterpart in the reference:
+ * it backs the decision on an MFMA path for N >= 4. tflops[0] / tflops[1] receive the
+ * useful FP32 rate (8 flops per complex multiply-add) of the VALU / MFMA form,
+ * max_rel_err[0] / [1] the error of each form against an FP64 host product
+ * over 1024 blocks. 2 <= N <= 8, 1 <= nrhs <= 32. Synchronous. */
+int qcuMrhsLinkBlockBench(int N, int nrhs, int reps, int resident, double *tflops, double *max_rel_err,
+                          void *stream);
 
 /* ---- solver ------------------------------------------------------------- */
 typedef struct qcuMrhsCgParam_s {

@@ -58,7 +58,8 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsGauge12Bytes", "qcuMrhsPackGauge12", "qcuMrhsDslash12",
                       "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
                       "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
-                      "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth", "qcuMrhsDslashExNoAlloc"]
+                      "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth", "qcuMrhsDslashExNoAlloc",
+                      "qcuMrhsLinkBlockBench"]
 
 _lib = None
 
@@ -119,6 +120,7 @@ def _declare(lib):
         "qcuMrhsCloverApply": (I, [P, P, P, I, I, I, G, P]),
         "qcuMrhsCgSolveClover": (I, [P, P, P, P, P, I, G, ctypes.POINTER(CgParam), ctypes.POINTER(D), P]),
         "qcuMrhsStreamBandwidth": (I, [I, P, P, ctypes.c_size_t, I, ctypes.POINTER(D), P]),
+        "qcuMrhsLinkBlockBench": (I, [I, I, I, I, ctypes.POINTER(D), ctypes.POINTER(D), P]),
     }
     # unpack takes a pointer array as its first argument
     sig["qcuMrhsUnpackSpinors"] = (I, [PP, I, P, I, I, G, P])
