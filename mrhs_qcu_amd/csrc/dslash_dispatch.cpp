@@ -78,6 +78,7 @@ Tile default_tile(int N, int k, int prec)
   if (half && k % 4 == 0 && N <= 3) t.R = 4;
   else if (k % 2 == 0 && N <= 5) t.R = 2;
   if (k / t.R >= 2) t.lds = 1;
+  if (prec == PREC_FP8) t.lds = 1;  // the FP8 kernels stage links in LDS
   return t;
 }
 
@@ -255,6 +256,10 @@ static int check_call(const DslashCall &c, std::string *err)
   if (c.parity != 0 && c.parity != 1) return fail(err, "parity must be 0 or 1"), 1;
   if (!bytes_of(c.prec_s) || !bytes_of(c.prec_g) || !bytes_of(c.prec_o))
     return fail(err, "unsupported storage precision"), 1;
+  if ((c.prec_s == PREC_FP8 || c.prec_o == PREC_FP8) &&
+      (c.sl != SL_PACKED || c.geom.N != 3 || c.k % 2 || c.tile.R < 2 || !c.tile.lds))
+    return fail(err, "FP8 storage is instantiated for the packed layout, nColors = 3, an even number of right-hand "
+                     "sides and tiles with rhs_per_lane 2 or 4 and LDS-staged links"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
   if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;

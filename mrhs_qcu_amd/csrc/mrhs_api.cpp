@@ -220,7 +220,8 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
           const int P = k / R;
           const size_t shm = size_t((block - 1) / P + 2) * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
           if (shm > size_t(kMaxDynLds)) continue;
-          if (P < 2) continue;  // no link reuse inside a workgroup
+          // No link reuse inside a workgroup; FP8 has LDS-staging kernels only.
+          if (P < 2 && prec != PREC_FP8) continue;
         }
         for (int tb : {1, 2, 4}) {
           if (g.L[3] % tb) continue;

@@ -131,3 +131,53 @@ def test_fp8_sloppy_cg(lib, dev):
     assert check_solution(g, b, x8, L, kappa, 1e-10) <= 2e-10
     d = max(np.linalg.norm(x8[i] - x64[i]) / np.linalg.norm(x64[i]) for i in range(k))
     assert d <= 1e-8, d
+
+
+@pytest.mark.parametrize("L,N,k", [((4, 4, 4, 8), 3, 3), ((4, 4, 4, 8), 4, 4)])
+def test_fp8_rejects_shapes_without_kernels(lib, dev, L, N, k):
+    """FP8 kernels exist for SU(3) with an even RHS count: other shapes fail
+    with a message, before any launch."""
+    geom = _lib.geom(L, N)
+    gn = torch.zeros(lib.qcuMrhsGaugeBytes(geom, _lib.FP8) // 2, dtype=torch.float16, device=dev)
+    pin = torch.zeros(lib.qcuMrhsSpinorBytes(geom, k, _lib.FP8), dtype=torch.uint8, device=dev)
+    pout = torch.empty_like(pin)
+    rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gn.data_ptr(), _lib.FP8, k, 0, 0, geom, None, stream())
+    assert rc != 0
+    assert b"FP8" in lib.qcuMrhsLastError()
+
+
+def test_fp8_even_rhs_counts(lib, dev):
+    """k = 2 (one lane per site) and k = 6 run with the default tile and match
+    the FP64 oracle to the FP8 error."""
+    L, N = (4, 4, 4, 8), 3
+    geom = _lib.geom(L, N)
+    g = fields.gauge_field(L, N, seed=31, smear_eps=0.3)
+    for k in (2, 6):
+        psi = fields.gaussian_spinors(L, N, k, seed=32 + k)
+        gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, _lib.FP8) // 2, dtype=torch.float16, device=dev)
+        _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), _lib.FP8, dev_tensor(g, dev).data_ptr(), _lib.DOUBLE, geom,
+                                        stream()))
+        ins = [dev_tensor(psi[r], dev) for r in range(k)]
+        arr, keep = _lib.ptr_array([t.data_ptr() for t in ins])
+        pin = torch.empty(lib.qcuMrhsSpinorBytes(geom, k, _lib.FP8), dtype=torch.uint8, device=dev)
+        pout = torch.empty_like(pin)
+        _lib.check(lib.qcuMrhsPackSpinors(pin.data_ptr(), _lib.FP8, arr, _lib.DOUBLE, k, geom, stream()))
+        _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gn.data_ptr(), _lib.FP8, k, 1, 0, geom, None,
+                                     stream()))
+        if k == 2:  # the tuner also finds a tile (one lane per site, links in LDS) and the result stays the same
+            t = _lib.Tile()
+            _lib.check(lib.qcuMrhsTune(pout.data_ptr(), pin.data_ptr(), gn.data_ptr(), _lib.FP8, k, geom,
+                                       ctypes.byref(t), None, stream()))
+            assert t.rhs_per_lane == 2 and t.link_lds == 1
+            first = pout.clone()
+            _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gn.data_ptr(), _lib.FP8, k, 1, 0, geom,
+                                         ctypes.byref(t), stream()))
+            torch.cuda.synchronize()
+            assert torch.equal(first, pout)
+        outs = [torch.empty(psi.shape[1:] + (2,), dtype=torch.float64, device=dev) for _ in range(k)]
+        oarr, okeep = _lib.ptr_array([t.data_ptr() for t in outs])
+        _lib.check(lib.qcuMrhsUnpackSpinors(oarr, _lib.DOUBLE, pout.data_ptr(), _lib.FP8, k, geom, stream()))
+        torch.cuda.synchronize()
+        err = rel_err(np.stack([host_complex(t) for t in outs]), oracle.dslash(g, psi, L, 1, 0))
+        print(f"FP8 k={k}: error vs FP64 oracle {err:.3e}")
+        assert err < 0.1
