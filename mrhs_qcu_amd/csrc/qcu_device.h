@@ -109,12 +109,15 @@ template <> struct StCvt<c16> {
   }
 };
 template <> struct StCvt<c8> {
-  // Round to nearest even, saturating at +-448 (the format has no infinity;
-  // a value the hardware would turn into NaN is clamped first).
+  // Round to nearest even, saturating at +-448: the format has no infinity, so
+  // finite values beyond it and +-inf are clamped first (the conversion would
+  // make them NaN). The clamp is written with comparisons so that a NaN stays
+  // NaN (fminf / fmaxf would turn it into -448).
+  __device__ __forceinline__ static float sat(float x) { return x > kFp8Max ? kFp8Max : (x < -kFp8Max ? -kFp8Max : x); }
   template <class A> __device__ __forceinline__ static c8 go(cpx<A> z)
   {
-    const float re = __builtin_fminf(__builtin_fmaxf(float(z.re), -kFp8Max), kFp8Max);
-    const float im = __builtin_fminf(__builtin_fmaxf(float(z.im), -kFp8Max), kFp8Max);
+    const float re = sat(float(z.re));
+    const float im = sat(float(z.im));
     const uint32_t w = uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(re, im, 0, false));
     c8 r;
     r.x = uint8_t(w & 0xffu);

@@ -87,6 +87,23 @@ int transpose_batched(void *dst, const void *src, int rows, int cols, int nb, in
 }
 
 // ---------------------------------------------------------------------------
+// FP64 -> FP32 rounded to odd (truncate toward zero, then set the last
+// mantissa bit if the result is inexact). A second rounding of that value to
+// a format with at least two fewer significand bits (FP8 e4m3 here) then
+// equals the direct correctly rounded conversion, which plain nearest
+// rounding to FP32 first does not guarantee (a double just past an FP8
+// midpoint can round to the midpoint in FP32 and then tie the wrong way).
+__device__ __forceinline__ float f64_to_f32_odd(double d)
+{
+  float f = float(d);
+  if (double(f) != d && d == d) {
+    // one step toward zero (|f| > 0 here; sign-magnitude bits: magnitude - 1)
+    if (__builtin_fabs(double(f)) > __builtin_fabs(d)) f = __uint_as_float(__float_as_uint(f) - 1u);
+    f = __uint_as_float(__float_as_uint(f) | 1u);
+  }
+  return f;
+}
+
 // Precision-converting element copy helpers.
 template <class CD, class CSrc>
 __device__ __forceinline__ CD cvt_elem(const CSrc &v)
@@ -95,7 +112,9 @@ __device__ __forceinline__ CD cvt_elem(const CSrc &v)
     return v;  // FP64 -> FP64: bit copy
   } else {
     using A = typename std::conditional<sizeof(CD) == 16, double, float>::type;
-    if constexpr (std::is_same<CD, c16>::value && std::is_same<CSrc, c64>::value) {
+    if constexpr (std::is_same<CD, c8>::value && std::is_same<CSrc, c64>::value) {
+      return StCvt<c8>::go(cpx<float>{f64_to_f32_odd(v.x), f64_to_f32_odd(v.y)});
+    } else if constexpr (std::is_same<CD, c16>::value && std::is_same<CSrc, c64>::value) {
       // Direct double -> binary16 rounding (one rounding step).
       c16 r;
       r.x = _Float16(v.x);

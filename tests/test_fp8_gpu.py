@@ -78,6 +78,14 @@ def test_fp8_conversion_matches_format(lib, dev):
     want = e4m3_encode(x)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, [(float(x[i]), int(got[i]), int(want[i])) for i in bad[:10]]
+    # NaN stays NaN (a NaN code, either sign); +-inf saturate to +-448
+    sp = torch.tensor([float("nan"), float("inf"), -float("inf"), float("nan")], dtype=torch.float32, device=dev)
+    dp = torch.zeros(4, dtype=torch.uint8, device=dev)
+    _lib.check(lib.qcuMrhsConvert(dp.data_ptr(), _lib.FP8, sp.data_ptr(), _lib.SINGLE, 2, stream()))
+    torch.cuda.synchronize()
+    sc = dp.cpu().numpy()
+    assert (sc[0] & 0x7F) == 0x7F and (sc[3] & 0x7F) == 0x7F, sc
+    assert sc[1] == 0x7E and sc[2] == 0xFE, sc
     # decode every finite code
     codes = np.array([c for c in range(256) if not np.isnan(vals[c])], dtype=np.uint8)
     codes = codes[: codes.size // 2 * 2]
@@ -86,6 +94,27 @@ def test_fp8_conversion_matches_format(lib, dev):
     _lib.check(lib.qcuMrhsConvert(back.data_ptr(), _lib.SINGLE, cd.data_ptr(), _lib.FP8, codes.size // 2, stream()))
     torch.cuda.synchronize()
     assert np.array_equal(back.cpu().numpy(), vals[codes].astype(np.float32))
+
+
+def test_fp8_from_fp64_rounds_once(lib, dev):
+    """FP64 -> FP8 is one correct rounding: values a hair (2^-30 relative)
+    past every midpoint, which FP32 cannot resolve, still round away from
+    the midpoint; exact midpoints tie to the even code."""
+    vals = e4m3_values()
+    pos = vals[:127]
+    mids = (pos[:-1] + pos[1:]) / 2
+    eps = 2.0 ** -30
+    x = np.concatenate([mids * (1 + eps), mids * (1 - eps), -mids * (1 + eps), mids, pos[1:] * (1 + eps)])
+    if x.size % 2:
+        x = np.append(x, 0.0)
+    src = torch.from_numpy(x.astype(np.float64)).to(dev)
+    dst = torch.zeros(x.size, dtype=torch.uint8, device=dev)
+    _lib.check(lib.qcuMrhsConvert(dst.data_ptr(), _lib.FP8, src.data_ptr(), _lib.DOUBLE, x.size // 2, stream()))
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    want = e4m3_encode(x)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, [(float(x[i]), int(got[i]), int(want[i])) for i in bad[:10]]
 
 
 def test_fp8_dslash_error_stated(lib, dev):
