@@ -89,7 +89,7 @@ int transpose_batched(void *dst, const void *src, int rows, int cols, int nb, in
 // ---------------------------------------------------------------------------
 // FP64 -> FP32 rounded to odd (truncate toward zero, then set the last
 // mantissa bit if the result is inexact). A second rounding of that value to
-// a format with at least two fewer significand bits (FP8 e4m3 here) then
+// a format with at least two fewer significand bits (FP8 e4m3, bfloat16) then
 // equals the direct correctly rounded conversion, which plain nearest
 // rounding to FP32 first does not guarantee (a double just past an FP8
 // midpoint can round to the midpoint in FP32 and then tie the wrong way).
@@ -112,8 +112,8 @@ __device__ __forceinline__ CD cvt_elem(const CSrc &v)
     return v;  // FP64 -> FP64: bit copy
   } else {
     using A = typename std::conditional<sizeof(CD) == 16, double, float>::type;
-    if constexpr (std::is_same<CD, c8>::value && std::is_same<CSrc, c64>::value) {
-      return StCvt<c8>::go(cpx<float>{f64_to_f32_odd(v.x), f64_to_f32_odd(v.y)});
+    if constexpr ((std::is_same<CD, c8>::value || std::is_same<CD, cb16>::value) && std::is_same<CSrc, c64>::value) {
+      return StCvt<CD>::go(cpx<float>{f64_to_f32_odd(v.x), f64_to_f32_odd(v.y)});
     } else if constexpr (std::is_same<CD, c16>::value && std::is_same<CSrc, c64>::value) {
       // Direct double -> binary16 rounding (one rounding step).
       c16 r;

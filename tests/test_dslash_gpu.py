@@ -5,6 +5,7 @@ FP64 1e-13, FP32 1e-6; FP16 / BF16 storage with FP32 arithmetic are stated
 (printed) and bounded loosely (3e-3 / 2e-2).
 """
 import ctypes
+import math
 import itertools
 
 import numpy as np
@@ -220,6 +221,46 @@ def test_pack_roundtrip_and_gauge_layout(lib, dev):
         want = fields.to_packed(psi)
         assert np.array_equal(host[..., 0], want.real.astype(npt))
         assert np.array_equal(host[..., 1], want.imag.astype(npt))
+    # BF16 packing from FP64 is one correct rounding (nearest, ties to even)
+    _, _, p = pack_inputs(lib, dev, g, psi, L, N, _lib.BF16)
+    bits = p.detach().view(torch.int16).cpu().numpy().view(np.uint16)
+    want = fields.to_packed(psi)
+    assert np.array_equal(bits[..., 0], bf16_bits(want.real))
+    assert np.array_equal(bits[..., 1], bf16_bits(want.imag))
+
+
+def bf16_round(x):
+    """Correctly rounded bfloat16 value of a double (8 significant bits,
+    nearest, ties to even; the normal range of FP32), by exact scaling."""
+    if x == 0 or not math.isfinite(x):
+        return x
+    m, e = math.frexp(x)  # x = m 2^e, 0.5 <= |m| < 1
+    ulp = math.ldexp(1.0, e - 8)
+    return round(x / ulp) * ulp  # x / ulp is exact; round() ties to even
+
+
+def bf16_bits(a):
+    a = np.asarray(a, dtype=np.float64)
+    r = np.array([bf16_round(float(v)) for v in a.ravel()]).reshape(a.shape)
+    return (r.astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def test_bf16_from_fp64_rounds_once(lib, dev):
+    """FP64 -> BF16: values a hair (2^-30 relative) past a bfloat16 midpoint,
+    which FP32 cannot tell from the midpoint, still round away from it;
+    exact midpoints tie to even."""
+    j = np.arange(0, 128)
+    mids = np.concatenate([(1 + (j + 0.5) / 128) * s for s in (1.0, 2.0 ** -9, 2.0 ** 13)])
+    eps = 2.0 ** -30
+    x = np.concatenate([mids * (1 + eps), mids * (1 - eps), -mids * (1 + eps), mids])
+    src = torch.from_numpy(x).to(dev)
+    dst = torch.zeros(x.size, dtype=torch.int16, device=dev)
+    _lib.check(lib.qcuMrhsConvert(dst.data_ptr(), _lib.BF16, src.data_ptr(), _lib.DOUBLE, x.size // 2, stream()))
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy().view(np.uint16)
+    want = bf16_bits(x)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, [(float(x[i]), int(got[i]), int(want[i])) for i in bad[:10]]
 
 
 def test_user_layout_path(lib, dev):
