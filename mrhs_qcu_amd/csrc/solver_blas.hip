@@ -31,11 +31,41 @@ __device__ __forceinline__ void block_partial(double v, int rhs, int k, double *
   for (int i = threadIdx.x; i < k; i += blockDim.x) partial[size_t(blockIdx.x) * k + i] = red[i];
 }
 
+// block_partial for V consecutive right-hand sides rhs0 .. rhs0 + V - 1.
+template <int V>
+__device__ __forceinline__ void block_partial_v(const double (&v)[V], int rhs0, int k, double *partial)
+{
+  __shared__ double red[kBlasMaxRhs];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < V; ++j) atomicAdd(&red[rhs0 + j], v[j]);
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) partial[size_t(blockIdx.x) * k + i] = red[i];
+}
+
+// V consecutive complex numbers of a packed field, one vector access.
+template <class C, int V>
+__device__ __forceinline__ CVec<C, V> ldv(const C *p, long long e)
+{
+  return *reinterpret_cast<const CVec<C, V> *>(p + e);
+}
+template <class C, int V>
+__device__ __forceinline__ void stv(C *p, long long e, const CVec<C, V> &v)
+{
+  *reinterpret_cast<CVec<C, V> *>(p + e) = v;
+}
+
 // x += alpha p (x null: skipped) ; p = gamma r + beta p (gamma null: 1) ;
 // p_s = p / sc_p (sloppy copy, may be null) ; partial |p|^2. With p null the
 // search direction lives only in p_s: the old p is p_s * sc_old (null: 1).
-template <class CK, class CS>
-__global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p, CS *__restrict__ ps,
+// The old direction is read from p_src / ps_src and the new one written to p
+// / ps (out of place, or the same buffer: each element is read before it is
+// written by the same lane). A lane takes V consecutive complex numbers per
+// step (V right-hand sides of one site row: k % V == 0), so a 16-bit field
+// moves in 16-byte accesses.
+template <class CK, class CS, int V>
+__global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *p, const CK *p_src, CS *ps, const CS *ps_src,
                                                               const CK *__restrict__ r,
                                                               const double *__restrict__ beta,
                                                               const double *__restrict__ gamma,
@@ -47,36 +77,102 @@ __global__ void __launch_bounds__(kBlasBlock) update_p_kernel(CK *__restrict__ p
   using A = typename Acc<CK>::T;
   const long long T = (long long)gridDim.x * blockDim.x;
   const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = int(g % k);
-  const A b = A(beta[rhs]);
-  const A c = gamma ? A(gamma[rhs]) : A(1);
-  const A al = x ? A(alpha[rhs]) : A(0);
-  const float inv = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
-  const A so = sc_old ? A(sc_old[rhs]) : A(1);
-  double acc = 0.0;
-  for (long long e = g; e < n; e += T) {
-    cpx<A> pp;
-    if (p) {
-      pp = ld_cvt<A>(p[e]);
-    } else {
-      pp = ld_cvt<A>(ps[e]);
-      pp.re *= so;
-      pp.im *= so;
-    }
-    const cpx<A> rr = ld_cvt<A>(r[e]);
-    if (x) {
-      cpx<A> xx = ld_cvt<A>(x[e]);
-      xx.re += al * pp.re;
-      xx.im += al * pp.im;
-      x[e] = StCvt<CK>::go(xx);
-    }
-    pp.re = c * rr.re + b * pp.re;
-    pp.im = c * rr.im + b * pp.im;
-    if (p) p[e] = StCvt<CK>::go(pp);
-    if (ps) ps[e] = StCvt<CS>::go(cpx<float>{float(pp.re) * inv, float(pp.im) * inv});
-    acc += double(pp.re) * pp.re + double(pp.im) * pp.im;
+  const int rhs0 = int((g * V) % k);  // constant along the walk: T is a multiple of k
+  A b[V], c[V], al[V], so[V];
+  float inv[V];
+  double acc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    const int rhs = rhs0 + j;
+    b[j] = A(beta[rhs]);
+    c[j] = gamma ? A(gamma[rhs]) : A(1);
+    al[j] = x ? A(alpha[rhs]) : A(0);
+    inv[j] = sc_p ? 1.0f / sc_p[rhs] : 1.0f;
+    so[j] = sc_old ? A(sc_old[rhs]) : A(1);
+    acc[j] = 0.0;
   }
-  block_partial(acc, rhs, k, partial);
+  const long long ng = n / V;
+  for (long long gi = g; gi < ng; gi += T) {
+    const long long e = gi * V;
+    cpx<A> pp[V];
+    if (p_src) {
+      const CVec<CK, V> v = ldv<CK, V>(p_src, e);
+#pragma unroll
+      for (int j = 0; j < V; ++j) pp[j] = ld_cvt<A>(v.v[j]);
+    } else {
+      const CVec<CS, V> v = ldv<CS, V>(ps_src, e);
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        pp[j] = ld_cvt<A>(v.v[j]);
+        pp[j].re *= so[j];
+        pp[j].im *= so[j];
+      }
+    }
+    const CVec<CK, V> rv = ldv<CK, V>(r, e);
+    if (x) {
+      CVec<CK, V> xv = ldv<CK, V>(x, e);
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        cpx<A> xx = ld_cvt<A>(xv.v[j]);
+        xx.re += al[j] * pp[j].re;
+        xx.im += al[j] * pp[j].im;
+        xv.v[j] = StCvt<CK>::go(xx);
+      }
+      stv<CK, V>(x, e, xv);
+    }
+    CVec<CK, V> po;
+    CVec<CS, V> pso;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const cpx<A> rr = ld_cvt<A>(rv.v[j]);
+      pp[j].re = c[j] * rr.re + b[j] * pp[j].re;
+      pp[j].im = c[j] * rr.im + b[j] * pp[j].im;
+      po.v[j] = StCvt<CK>::go(pp[j]);
+      pso.v[j] = StCvt<CS>::go(cpx<float>{float(pp[j].re) * inv[j], float(pp[j].im) * inv[j]});
+      acc[j] += double(pp[j].re) * pp[j].re + double(pp[j].im) * pp[j].im;
+    }
+    if (p) stv<CK, V>(p, e, po);
+    if (ps) stv<CS, V>(ps, e, pso);
+  }
+  block_partial_v<V>(acc, rhs0, k, partial);
+}
+
+// xh += sum_j alpha_j sc_j p_j over the stored search directions of the steps
+// not yet applied (sc_j null: 1), in FP64, in step order; V consecutive
+// complex numbers per lane and step, as update_p.
+template <class CP, int V>
+__global__ void __launch_bounds__(kBlasBlock) flush_kernel(c64 *__restrict__ xh, const DirSteps steps, long long n,
+                                                           int k)
+{
+  const long long T = (long long)gridDim.x * blockDim.x;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int rhs0 = int((g * V) % k);
+  double coef[kMaxDirSteps][V];
+#pragma unroll
+  for (int s = 0; s < kMaxDirSteps; ++s)
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      coef[s][j] = s < steps.count
+                       ? steps.alpha[s][rhs0 + j] * (steps.sc[s] ? double(steps.sc[s][rhs0 + j]) : 1.0)
+                       : 0.0;
+  const long long ng = n / V;
+  for (long long gi = g; gi < ng; gi += T) {
+    const long long e = gi * V;
+    CVec<c64, V> h = ldv<c64, V>(xh, e);
+#pragma unroll
+    for (int s = 0; s < kMaxDirSteps; ++s) {
+      if (s < steps.count) {
+        const CVec<CP, V> pv = ldv<CP, V>(static_cast<const CP *>(steps.p[s]), e);
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          const cpx<double> pp = ld_cvt<double>(pv.v[j]);
+          h.v[j].x += coef[s][j] * pp.re;
+          h.v[j].y += coef[s][j] * pp.im;
+        }
+      }
+    }
+    stv<c64, V>(xh, e, h);
+  }
 }
 
 // partial |v|^2
@@ -126,43 +222,6 @@ __global__ void __launch_bounds__(kBlasBlock) residual_kernel(CK *r, const c64 *
   block_partial(acc, rhs, k, partial_rr);
   __syncthreads();
   block_partial(dot, rhs, k, partial_rp);
-}
-
-// xh += x + alpha p (x, p nullable; the sum formed in x's precision, then
-// converted to FP64), then x = 0
-template <class CK, class CP>
-__global__ void __launch_bounds__(kBlasBlock) accumulate_kernel(c64 *__restrict__ xh, CK *__restrict__ x,
-                                                                const CP *__restrict__ p,
-                                                                const float *__restrict__ sc_p,
-                                                                const double *__restrict__ alpha, long long n, int k)
-{
-  const long long T = (long long)gridDim.x * blockDim.x;
-  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  using A = typename Acc<CK>::T;
-  const int rhs = int(g % k);
-  // alpha times the true p (stored p times its scale)
-  const A al = p ? A(alpha[rhs]) * (sc_p ? A(sc_p[rhs]) : A(1)) : A(0);
-  for (long long e = g; e < n; e += T) {
-    cpx<double> v{0.0, 0.0};
-    if (x) {
-      // The step lands in x_k's precision first, as it would in x_k.
-      cpx<A> xx = ld_cvt<A>(x[e]);
-      if (p) {
-        const cpx<A> pp = ld_cvt<A>(p[e]);
-        xx.re += al * pp.re;
-        xx.im += al * pp.im;
-      }
-      v = {double(xx.re), double(xx.im)};
-      x[e] = StCvt<CK>::go(cpx<double>{0.0, 0.0});
-    } else if (p) {
-      const cpx<A> pp = ld_cvt<A>(p[e]);
-      v = {double(al * pp.re), double(al * pp.im)};
-    }
-    c64 h = xh[e];
-    h.x += v.re;
-    h.y += v.im;
-    xh[e] = h;
-  }
 }
 
 // Fill with zeros (a kernel, so the solver loop stays capturable in a graph).
@@ -317,14 +376,20 @@ long long blas_num_blocks(long long n, int k) { return grid_for(n, k); }
 
 #define QM_GRID(n, k) dim3(grid_for(n, k)), dim3(kBlasBlock), 0, st
 
-hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *r, const double *beta,
-                         const double *gamma, void *x, const double *alpha, const float *sc_p, const float *sc_old,
-                         double *partial, long long n, int k, hipStream_t st)
+hipError_t blas_update_p(int prec_k, int prec_s, void *p, const void *p_src, void *ps, const void *ps_src,
+                         const void *r, const double *beta, const double *gamma, void *x, const double *alpha,
+                         const float *sc_p, const float *sc_old, double *partial, long long n, int k, hipStream_t st)
 {
-#define QM_UP(CK, CS)                                                                                          \
-  hipLaunchKernelGGL((update_p_kernel<CK, CS>), QM_GRID(n, k), static_cast<CK *>(p), static_cast<CS *>(ps),    \
-                     static_cast<const CK *>(r), beta, gamma, static_cast<CK *>(x), alpha, sc_p, sc_old, partial, \
-                     n, k)
+#define QM_UP_V(CK, CS, V)                                                                                          \
+  hipLaunchKernelGGL((update_p_kernel<CK, CS, V>), QM_GRID(n, k), static_cast<CK *>(p),                           \
+                     static_cast<const CK *>(p_src), static_cast<CS *>(ps), static_cast<const CS *>(ps_src),          \
+                     static_cast<const CK *>(r), beta, gamma, static_cast<CK *>(x), alpha, sc_p, sc_old, partial, n, k)
+#define QM_UP(CK, CS)          \
+  if (k % 4 == 0) {            \
+    QM_UP_V(CK, CS, 4);        \
+  } else {                     \
+    QM_UP_V(CK, CS, 1);        \
+  }
   if (prec_k == PREC_DOUBLE) {
     QM_UP(c64, c64);
   } else {
@@ -336,6 +401,7 @@ hipError_t blas_update_p(int prec_k, int prec_s, void *p, void *ps, const void *
     }
   }
 #undef QM_UP
+#undef QM_UP_V
   return hipGetLastError();
 }
 
@@ -388,27 +454,22 @@ hipError_t blas_residual(int prec_k, void *r, const void *b, const void *Ax, int
   return hipGetLastError();
 }
 
-hipError_t blas_accumulate(int prec_k, void *xh, void *x, const void *p, int prec_p, const float *sc_p,
-                           const double *alpha, long long n, int k, hipStream_t st)
+hipError_t blas_flush(int prec_p, void *xh, const DirSteps &steps, long long n, int k, hipStream_t st)
 {
-#define QM_ACC(CK, CP)                                                                                            \
-  hipLaunchKernelGGL((accumulate_kernel<CK, CP>), QM_GRID(n, k), static_cast<c64 *>(xh), static_cast<CK *>(x), \
-                     static_cast<const CP *>(p), sc_p, alpha, n, k)
-#define QM_ACC_P(CK)                           \
-  switch (prec_p) {                            \
-    case PREC_DOUBLE: QM_ACC(CK, c64); break;   \
-    case PREC_SINGLE: QM_ACC(CK, c32); break;   \
-    case PREC_HALF: QM_ACC(CK, c16); break;     \
-    case PREC_FP8: QM_ACC(CK, c8); break;       \
-    default: QM_ACC(CK, cb16);                  \
+  if (steps.count < 1) return hipSuccess;
+  if (steps.count > kMaxDirSteps) return hipErrorInvalidValue;
+  c64 *x = static_cast<c64 *>(xh);
+#define QM_FL(CP)                                                                          \
+  if (k % 4 == 0) hipLaunchKernelGGL((flush_kernel<CP, 4>), QM_GRID(n, k), x, steps, n, k); \
+  else hipLaunchKernelGGL((flush_kernel<CP, 1>), QM_GRID(n, k), x, steps, n, k)
+  switch (prec_p) {
+    case PREC_DOUBLE: QM_FL(c64); break;
+    case PREC_SINGLE: QM_FL(c32); break;
+    case PREC_HALF: QM_FL(c16); break;
+    case PREC_FP8: QM_FL(c8); break;
+    default: QM_FL(cb16);
   }
-  if (prec_k == PREC_DOUBLE) {
-    QM_ACC_P(c64)
-  } else {
-    QM_ACC_P(c32)
-  }
-#undef QM_ACC_P
-#undef QM_ACC
+#undef QM_FL
   return hipGetLastError();
 }
 

@@ -9,12 +9,17 @@
 //   * dslash storage in sloppy_prec (FP16 / BF16 / FP32) with FP32 arithmetic;
 //     binary16 fields carry a per-RHS scale (true = stored * scale) chosen
 //     from the CG scalars so no value can overflow;
-//   * Krylov vectors r, p, x_k and M^+M^ p in FP32;
+//   * the residual r and M^+M^ p in FP32; the search direction only in its
+//     (scaled) sloppy copy, which the operator reads;
 //   * all dot products and norms accumulated in FP64 (per-workgroup partials,
 //     deterministic reduction);
-//   * every reliable_interval iterations (and whenever the iterated residual
-//     says converged) x is folded into the FP64 solution and the residual is
-//     recomputed in FP64 from the FP64 operator.
+//   * the solution is FP64 only: the search directions of the last few steps
+//     stay stored (a ring of kDirRing fields) with their step lengths, and
+//     x += sum alpha_j p_j is applied in FP64 at each reliable update or when
+//     the ring is full, so no per-iteration x field is read and rewritten;
+//   * the true residual is recomputed from the FP64 operator when the
+//     iterated residual fell by delta, every reliable_interval iterations and
+//     whenever the iterated residual says converged.
 // With sloppy_prec == DOUBLE every field is FP64 and the true residual is
 // recomputed only when the iterated residual reaches the tolerance.
 // All scalars live on the device; the host reads k flags per check.
@@ -38,6 +43,12 @@
 #include "storage_api.h"
 
 namespace qmrhs {
+
+// Search directions kept between x updates (each one sloppy field, FP64 in the
+// FP64 solve): x += sum alpha_j p_j is applied once per ring instead of
+// reading and rewriting an x field every iteration.
+constexpr int kDirRing = 8;
+static_assert(kDirRing <= kMaxDirSteps, "a flush applies up to a whole ring of steps");
 
 int qcu_context_colors();
 hipError_t cg_init_state(const CgState &s, int k, hipStream_t st);
@@ -269,11 +280,25 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   void *ax64 = mem.bytes(b64, err, &ok);
   void *r = mem.bytes(bk, err, &ok);
   // The search direction: FP64 in the FP64 solve; in a mixed solve it lives
-  // only in its (scaled) sloppy copy p_s, which the operator reads, so x and r
-  // advance along exactly the direction the operator was applied to.
-  void *p = mixed ? nullptr : mem.bytes(bk, err, &ok);
-  void *xk = mixed ? mem.bytes(bk, err, &ok) : x_out;
-  void *p_s = mixed ? mem.bytes(bs, err, &ok) : p;
+  // only in its (scaled) sloppy copy, which the operator reads, so x and r
+  // advance along exactly the direction the operator was applied to. It is
+  // written out of place into a ring of stored directions: the x steps of the
+  // directions since the last update are applied together, in FP64.
+  const int dir_prec = mixed ? ps : pk;
+  const size_t bdir = mixed ? bs : bk;
+  int nring = kDirRing;
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+      // at most a quarter of the free memory, at least two directions
+      const size_t fit = free_b / 4 / std::max<size_t>(bdir, 1);
+      nring = int(std::max<size_t>(2, std::min<size_t>(size_t(kDirRing), fit)));
+    }
+  }
+  std::vector<void *> ring(nring);
+  for (int j = 0; j < nring; ++j) ring[j] = mem.bytes(bdir, err, &ok);
+  double *ring_alpha = mem.alloc<double>(size_t(nring) * k, err, &ok);
+  float *ring_sc = mem.alloc<float>(size_t(nring) * k, err, &ok);
   void *t_s = mixed ? mem.bytes(bs, err, &ok) : t64;
   void *q_s = mixed ? mem.bytes(bs, err, &ok) : ax64;
   void *t2_s = mixed ? mem.bytes(bs, err, &ok) : t64;
@@ -343,7 +368,7 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   }
   // Tune the operator tiles for this shape once per process (the tuner
   // overwrites the scratch fields it times on; they are initialised below).
-  if (!tuned(g, ps, k) && !tune_now(t_s, p_s, gs, ps, k, g, st, err)) return 1;
+  if (!tuned(g, ps, k) && !tune_now(t_s, ring[0], gs, ps, k, g, st, err)) return 1;
   if (mixed && !tuned(g, PREC_DOUBLE, k) && !tune_now(t64, ax64, gauge64, PREC_DOUBLE, k, g, st, err)) return 1;
   // Reliable updates taken while every live residual is still far above FP32
   // resolution (relative residual > 1e-5; the FP32 operator's error is about
@@ -369,10 +394,12 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   const float *sc_p = scaled ? cs.sc_p : nullptr, *sc_t = scaled ? cs.sc_t : nullptr;
   const float *sc_q = scaled ? cs.sc_q : nullptr, *sc_t2 = scaled ? cs.sc_t2 : nullptr;
   const float *sc_p_old = scaled ? cs.sc_p_old : nullptr;
-  // The stored search direction, its precision and scale (as read by the
-  // accumulation and the residual of a reliable update, before cg_beta).
-  void *pdir = mixed ? p_s : p;
-  const int pdir_prec = mixed ? ps : pk;
+  // The current search direction (ring slot cur), its precision and scale
+  // (as read by the operator and by the residual of a reliable update,
+  // before cg_beta); `pending` holds the slots whose x step is not applied.
+  int cur = 0;
+  std::vector<int> pending;
+  const int pdir_prec = dir_prec;
 
 #define QM_TRY(expr)                                     \
   do {                                                   \
@@ -387,16 +414,43 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   QM_HIP(cg_init_state(cs, k, st));  // rr = b2, beta = 0, done = 0, scales from |b^|
   // x = 0, r = b^, p = r
   QM_HIP(blas_zero(PREC_DOUBLE, x_out, n, k, st));
-  if (mixed) QM_HIP(blas_zero(pk, xk, n, k, st));
   if (mixed) {
     std::string e2;
     if (convert_complex(r, pk, bhat, PREC_DOUBLE, n, st, &e2)) return *err = e2, 1;
   } else {
     QM_HIP(hipMemcpyAsync(r, bhat, b64, hipMemcpyDeviceToDevice, st));
   }
-  QM_HIP(blas_zero(pdir_prec, pdir, n, k, st));
-  QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, nullptr, nullptr, sc_p, sc_p_old,
-                       partial_p, n, k, st));
+  QM_HIP(blas_zero(pdir_prec, ring[cur], n, k, st));
+  QM_HIP(blas_update_p(pk, ps, mixed ? nullptr : ring[cur], mixed ? nullptr : ring[cur], mixed ? ring[cur] : nullptr,
+                       mixed ? ring[cur] : nullptr, r, cs.beta, nullptr, nullptr, nullptr, sc_p, sc_p_old, partial_p, n,
+                       k, st));
+  // x += sum of the pending steps (and, with `step`, the step along the
+  // current direction), in FP64.
+  auto flush_steps = [&](bool step) -> bool {
+    DirSteps ds;
+    auto add = [&](int slot) {
+      ds.p[ds.count] = ring[slot];
+      ds.alpha[ds.count] = ring_alpha + size_t(slot) * k;
+      ds.sc[ds.count] = scaled ? ring_sc + size_t(slot) * k : nullptr;
+      ++ds.count;
+    };
+    for (int slot : pending) add(slot);
+    if (step) add(cur);
+    pending.clear();
+    return hip_ok(blas_flush(pdir_prec, x_out, ds, n, k, st), err, "flush");
+  };
+  // p_new = gamma r + beta p into the next ring slot (gamma null: 1).
+  auto next_direction = [&](const double *gamma) -> bool {
+    const int nxt = (cur + 1) % nring;
+    void *src = ring[cur], *dst = ring[nxt];
+    if (!hip_ok(blas_update_p(pk, ps, mixed ? nullptr : dst, mixed ? nullptr : src, mixed ? dst : nullptr,
+                              mixed ? src : nullptr, r, cs.beta, gamma, nullptr, nullptr, sc_p, sc_p_old, partial_p, n,
+                              k, st),
+                err, "update_p"))
+      return false;
+    cur = nxt;
+    return true;
+  };
 
   std::vector<int> done(k, 0);
   std::vector<double> rrt(k, 0.0), b2h(k, 0.0), rr_ru(k, 0.0);
@@ -427,14 +481,13 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   QM_HIP(hipHostMalloc(&pin.p, size_t(k) * sizeof(double), hipHostMallocDefault));
   double *h_rr = static_cast<double *>(pin.p);
 
-  // True residual: x_h += x_k (mixed) + alpha p (the step of the iteration
-  // just finished, when `step`), r_h = b^ - M^+M^ x_h, |r_h|^2 replaces the
-  // iterated norm, done flags; the Krylov residual restarts from r_h.
+  // True residual: x_h += the pending steps and, with `step`, alpha p (the
+  // step of the iteration just finished), r_h = b^ - M^+M^ x_h, |r_h|^2
+  // replaces the iterated norm, done flags; the Krylov residual restarts
+  // from r_h.
   auto reliable_update = [&](bool step, bool use32) -> bool {
-    if ((mixed || step) && !hip_ok(blas_accumulate(pk, x_out, mixed ? xk : nullptr, step ? pdir : nullptr,
-                                                   pdir_prec, sc_p, cs.alpha, n, k, st),
-                                   err, "accumulate"))
-      return false;
+    if (!flush_steps(step)) return false;
+    void *pdir = ring[cur];
     // r = b^ - M^+M^ x_h straight into the Krylov vector, with |r|^2 and
     // <r, p> (for re-orthogonalising p) from the same pass.
     if (use32) {
@@ -484,8 +537,8 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
   for (int i = 0; i < k; ++i) converged = converged && done[i];
 
   // r -= alpha Ap is fused into the last operator launch of an iteration;
-  // x += alpha p into the next p update (which reads the old p anyway), or
-  // into the accumulation of a reliable update.
+  // x += alpha p waits in the ring (the direction stays stored with its alpha
+  // and scale) until a reliable update or a full ring applies it in FP64.
   DslashCall fuse;
   fuse.rx_r = r;
   fuse.rx_alpha = cs.alpha;
@@ -493,16 +546,21 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
     // t = D_oe p ; q = p - kappa^2 D_eo t (|q|^2 = <p, M^+M^ p>). Clover:
     // q = A_ee p - kappa^2 D_eo A_oo^-1 t (the clover applications keep the
     // per-RHS storage scale of their input, the operator being linear).
-    QM_TRY(s.dslash(t_s, p_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
+    void *p_cur = ring[cur];
+    QM_TRY(s.dslash(t_s, p_cur, nullptr, 0.0, 1.0, gs, ps, ps, 1, 0, sc_p, nullptr, sc_t, false));
     if (clov) {
-      QM_TRY(s.clover(u_s, t_s, cli_s, ps, 1) && s.clover(a_s, p_s, cl_s, ps, 0));
+      QM_TRY(s.clover(u_s, t_s, cli_s, ps, 1) && s.clover(a_s, p_cur, cl_s, ps, 0));
       QM_TRY(s.dslash(q_s, u_s, a_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
     } else {
-      QM_TRY(s.dslash(q_s, t_s, p_s, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
+      QM_TRY(s.dslash(q_s, t_s, p_cur, 1.0, kk2, gs, ps, ps, 0, 0, sc_t, sc_p, sc_q, true));
     }
     QM_HIP(blas_reduce(s.partial, s.dslash_blocks(ps), k, cs.pap, stage, st));
     QM_HIP(blas_reduce(partial_p, nb_blas, k, cs.pnorm2, stage, st));
     QM_HIP(cg_alpha(cs, k, st));
+    // this step's alpha and the direction's storage scale, kept with its slot
+    QM_HIP(hipMemcpyAsync(ring_alpha + size_t(cur) * k, cs.alpha, k * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (scaled)
+      QM_HIP(hipMemcpyAsync(ring_sc + size_t(cur) * k, cs.sc_p, k * sizeof(float), hipMemcpyDeviceToDevice, st));
     // Ap = q - kappa^2 D^+_eo D^+_oe q
     QM_TRY(s.dslash(t2_s, q_s, nullptr, 0.0, 1.0, gs, ps, ps, 1, 1, sc_q, nullptr, sc_t2, false));
     // Ap = q - kappa^2 D^+_eo t2 stays in registers: the same launch does
@@ -536,17 +594,19 @@ int cg_solve(void *x_out, const void *b, const void *gauge64, int k, const Geom 
       // made orthogonal to r_true.
       QM_HIP(cg_beta(cs, k, stop2, st));
       QM_HIP(cg_orth(cs, k, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, cs.gamma, nullptr, nullptr, sc_p, sc_p_old,
-                           partial_p, n, k, st));
+      QM_TRY(next_direction(cs.gamma));
     } else {
-      // x += alpha p ; beta = |r|^2 / |r_old|^2 ; p = r + beta p
+      // beta = |r|^2 / |r_old|^2 ; p = r + beta p into the next slot; this
+      // step's x update waits in the ring (applied before the slot is reused)
       QM_HIP(cg_beta(cs, k, stop2, st));
-      QM_HIP(blas_update_p(pk, ps, p, mixed ? p_s : nullptr, r, cs.beta, nullptr, xk, cs.alpha, sc_p, sc_p_old,
-                           partial_p, n, k, st));
+      pending.push_back(cur);
+      QM_TRY(next_direction(nullptr));
+      if (int(pending.size()) >= nring - 1) QM_TRY(flush_steps(false));
     }
   }
-  // Every iteration ends in a reliable update or in a p update that applied
-  // its x step, so x is complete here.
+  // Every iteration ends in a reliable update, which applies every pending x
+  // step, or leaves its step pending in the ring: apply what is left.
+  if (!pending.empty()) QM_TRY(flush_steps(false));
   if (!converged && since_update > 0) QM_TRY(reliable_update(false, false));
   QM_HIP(hipMemcpyAsync(rrt.data(), rr_true, k * sizeof(double), hipMemcpyDeviceToHost, st));
   QM_HIP(hipMemcpyAsync(b2h.data(), cs.b2, k * sizeof(double), hipMemcpyDeviceToHost, st));
