@@ -52,12 +52,17 @@ typedef struct qcuMrhsTile_s {
   int z_block;      /* ... and z_block z slices; (1, 0) = natural order, 0 = Lz */
   int cache_hints;  /* bit 0: non-temporal output stores; bit 1: non-temporal link loads;
                        bit 2: keep the hardware workgroup order (no XCD-contiguous remap);
-                       bit 3 (traversal 1 only): XCD-cyclic order, the 8 XCDs work side by side
+                       bit 3 (y-streamed traversals only): XCD-cyclic order, the 8 XCDs work side by side
                        on adjacent halves of y-streamed blocks (ignored when the lattice does
                        not split into whole chunks of workgroups) */
   int occupancy;    /* cap on resident workgroups per CU (3..8), 0 = no cap */
-  int traversal;    /* 0: each (t_block, z_block) block in (t, z, y, x) order;
-                       1: the block streamed along y, (y, t, z, x) order inside it */
+  int traversal;    /* bit 0 = 0: each (t_block, z_block) block in (t, z, y, x) order;
+                       bit 0 = 1: the block streamed along y, (y, t, z, x) order inside it;
+                       bit 1: hypercube mapping, one wavefront per 2x2x2x2 block of output
+                       sites (neighbours inside the block shared across lanes); needs
+                       P = nrhs / rhs_per_lane in {1, 2, 4, ..., 32}, block a multiple
+                       of 8 P, link_lds = 1, nColors <= 4,
+                       and t_block / z_block then count blocks (divisors of Lt/2, Lz/2) */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);
@@ -219,6 +224,10 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
                 qcuMrhsTile *best, double *best_us, void *stream);
 /* Tile the dispatcher uses for a shape (tuned if known, else the default). */
 int qcuMrhsGetTile(const qcuMrhsGeom *g, int prec, int nrhs, qcuMrhsTile *tile);
+/* Sets the tile the dispatcher, qcuMrhsDslashEx and the solver use for a
+ * shape in this process (also appended to the QCU_MRHS_TUNE_CACHE file when
+ * that is set). The tile is checked when it is next launched. */
+int qcuMrhsSetTile(const qcuMrhsGeom *g, int prec, int nrhs, const qcuMrhsTile *tile);
 /* Number of instantiated dslash kernels (for diagnostics). */
 int qcuMrhsKernelCount(void);
 

@@ -59,7 +59,7 @@ QCU_MRHS_H_SYMBOLS = ["qcuMrhsLastError", "qcuMrhsVersion", "qcuMrhsSpinorBytes"
                       "qcuMrhsGauge8Bytes", "qcuMrhsPackGauge8", "qcuMrhsDslash8",
                       "qcuMrhsCloverBytes", "qcuMrhsCloverCompute", "qcuMrhsCloverApply",
                       "qcuMrhsCgSolveClover", "qcuMrhsStreamBandwidth", "qcuMrhsDslashExNoAlloc",
-                      "qcuMrhsLinkBlockBench"]
+                      "qcuMrhsLinkBlockBench", "qcuMrhsSetTile"]
 
 _lib = None
 
@@ -99,6 +99,7 @@ def _declare(lib):
         "qcuMrhsDslashReferenceLayout": (I, [PP, PP, P, I, I, I, I, G]),
         "qcuMrhsTune": (I, [P, P, P, I, I, G, T, ctypes.POINTER(D), P]),
         "qcuMrhsGetTile": (I, [G, I, I, T]),
+        "qcuMrhsSetTile": (I, [G, I, I, T]),
         "qcuMrhsKernelCount": (I, []),
         "qcuMrhsCgSolve": (I, [P, P, P, I, G, ctypes.POINTER(CgParam), ctypes.POINTER(D), P]),
         "qcuMrhsSolverRelease": (I, []),

@@ -123,6 +123,7 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       t.lds = lds;
       t.block = kBatchBlock;
       t.occ = 0;
+      t.hcube = 0;  // the work-list body is the site-major kernel (the blocking still divides the lattice)
     }
   }
   if (!worklist)

@@ -170,10 +170,19 @@ unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k)
   if (!(t.hints & 8) || (t.hints & 4) || !t.ystream || t.R < 1 || k % t.R) return 0;
   const int B = t.block > 0 ? t.block : 256;
   const long long tb = t.tblock < 1 ? 1 : t.tblock;
-  const long long zb = t.zblock < 1 ? g.L[2] : t.zblock;
   const long long P = k / t.R;
   const long long lanes = 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3] * P;
-  const long long half = tb * zb * (g.L[1] / 2) * (g.L[0] / 2) * P;  // lanes per half block
+  long long half;  // lanes per half block
+  if (t.hcube) {
+    // Block lattice: 8 P lanes per 2x2x2x2 block, Ly / 2 blocks along y
+    // (a whole y line of blocks when that is odd).
+    const long long zb = t.zblock < 1 ? g.L[2] / 2 : t.zblock;
+    const long long by = g.L[1] / 2;
+    half = tb * zb * (by % 2 ? by : by / 2) * (g.L[0] / 2) * 8 * P;
+  } else {
+    const long long zb = t.zblock < 1 ? g.L[2] : t.zblock;
+    half = tb * zb * (g.L[1] / 2) * (g.L[0] / 2) * P;
+  }
   if (half % B || lanes % B) return 0;
   const long long gran = half / B, nblocks = lanes / B;
   if (nblocks % (8 * gran)) return 0;
@@ -205,7 +214,40 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   if (c.gl == GL_NATIVE8 && !c.tile.lds)
     return fail(err, "eight-real links are rebuilt while staging: the tile needs link_lds = 1"), 1;
   const int lds = c.tile.lds ? 1 : 0;
+  int hc = c.tile.hcube ? 1 : 0;
+  if (hc) {
+    // A hypercube tile runs the site-major kernel of the same shape where no
+    // hypercube kernel is instantiated for these precisions / N / R (its
+    // blocking divides the lattice either way).
+    KernelKey kh{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
+    kh.hc = 1;
+    KernelKey kh_kf = kh;
+    kh_kf.kf = c.k;
+    if (!find_kernel(kh) && !find_kernel(kh_kf)) hc = 0;
+  }
+  if (hc) {
+    // 2x2x2x2 blocks of 8 P lanes (P = k / R a power of two <= 32; the
+    // workgroup holds whole blocks), native links staged in LDS; tblock /
+    // zblock count blocks.
+    const int P = c.k / R;
+    if (P > 32 || (P & (P - 1)))
+      return fail(err, "hypercube tiles need nrhs / rhs_per_lane in {1, 2, 4, 8, 16, 32}"), 1;
+    if (B % (8 * P)) return fail(err, "hypercube tiles need a block that is a multiple of 8 * nrhs / rhs_per_lane"), 1;
+    if (c.gl != GL_NATIVE || !lds) return fail(err, "hypercube tiles need native links staged in LDS"), 1;
+    const int bz = c.geom.L[2] / 2;
+    a.tblock = c.tile.tblock < 1 ? 1 : c.tile.tblock;
+    a.zblock = c.tile.zblock < 1 ? bz : c.tile.zblock;
+    a.bLy = c.geom.L[1] / 2;
+    a.bLz = bz;
+    a.div_tb = make_fastdiv(uint32_t(a.tblock));
+    a.div_zb = make_fastdiv(uint32_t(a.zblock));
+    a.hdiv_Ly = make_fastdiv(uint32_t(a.bLy));
+    a.hdiv_row = make_fastdiv(uint32_t(a.subLx * a.bLy));
+    a.hdiv_nzb = make_fastdiv(uint32_t(bz / a.zblock));
+    a.hc_lp = uint32_t(__builtin_ctz(unsigned(P)));
+  }
   KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
+  key.hc = hc;
   // The same kernel compiled for this exact RHS count, when one exists
   // (bit-identical results, constant row offsets).
   KernelKey key_kf = key;
@@ -220,7 +262,9 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
   a.nlanes = uint32_t(lanes);
   a.nblocks = uint32_t((lanes + B - 1) / B);
-  a.xcd_gran = xcd_cycle_gran(c.geom, c.tile, c.k);
+  Tile tl = c.tile;
+  tl.hcube = hc;
+  a.xcd_gran = xcd_cycle_gran(c.geom, tl, c.k);
   size_t shm = 0;
   if (lds) {
     const size_t nsite_max = size_t((B - 1) / a.P + 2);
@@ -262,8 +306,18 @@ static int check_call(const DslashCall &c, std::string *err)
                      "sides and tiles with rhs_per_lane 2 or 4 and LDS-staged links"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
-  if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
-  if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
+  if (!c.tile.hcube) {
+    if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
+    if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
+  } else {
+    // Counted in 2x2x2x2 blocks (so they also divide Lt and Lz when the
+    // site-major kernel runs the tile).
+    if (c.sl != SL_PACKED) return fail(err, "hypercube tiles are for the packed layout"), 1;
+    if (c.tile.tblock > 1 && (c.geom.L[3] / 2) % c.tile.tblock)
+      return fail(err, "hypercube t_block must divide Lt / 2"), 1;
+    if (c.tile.zblock > 0 && (c.geom.L[2] / 2) % c.tile.zblock)
+      return fail(err, "hypercube z_block must divide Lz / 2"), 1;
+  }
   if (!c.gauge) return fail(err, "null gauge"), 1;
   return 0;
 }

@@ -7,6 +7,7 @@
 
 #include <string>
 
+#include "dslash_hcube.h"
 #include "dslash_kernel.h"
 
 namespace qmrhs {
@@ -14,10 +15,11 @@ namespace qmrhs {
 struct KernelKey {
   int sl, gl, n, r, lds, site_minor, prec_s, prec_g, prec_o;
   int kf = 0;  // > 0: instantiated for exactly this RHS count
+  int hc = 0;  // 1: hypercube work mapping (dslash_hcube.h)
   bool operator==(const KernelKey &o) const
   {
     return sl == o.sl && gl == o.gl && n == o.n && r == o.r && lds == o.lds && site_minor == o.site_minor &&
-           prec_s == o.prec_s && prec_g == o.prec_g && prec_o == o.prec_o && kf == o.kf;
+           prec_s == o.prec_s && prec_g == o.prec_g && prec_o == o.prec_o && kf == o.kf && hc == o.hc;
   }
 };
 
@@ -53,6 +55,30 @@ struct Reg {
   }
 };
 
+template <int N, int R, class CS, class CG, class CO, class A, int KF>
+hipError_t launch_hc_impl(const DslashArgs &a, dim3 g, dim3 b, size_t shm, hipStream_t st)
+{
+  hipLaunchKernelGGL((dslash_hc_kernel<N, R, CS, CG, CO, A, KF>), g, b, shm, st, a);
+  return hipGetLastError();
+}
+template <int N, int R, class CS, class CG, class CO, class A, int KF>
+hipError_t attr_hc_impl(hipFuncAttributes *at)
+{
+  return hipFuncGetAttributes(at, reinterpret_cast<const void *>(&dslash_hc_kernel<N, R, CS, CG, CO, A, KF>));
+}
+
+// Hypercube kernels: packed layout, native links staged in LDS.
+template <int N, int R, class CS, class CG, class CO, class A, int KF = 0>
+struct RegHc {
+  RegHc()
+  {
+    KernelKey key{SL_PACKED, GL_NATIVE, N, R, 1, 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec, StoreInfo<CO>::prec};
+    key.kf = KF;
+    key.hc = 1;
+    register_kernel(key, &launch_hc_impl<N, R, CS, CG, CO, A, KF>, &attr_hc_impl<N, R, CS, CG, CO, A, KF>);
+  }
+};
+
 // Dynamic LDS a launch may request: 64 KiB per workgroup minus the kernel's
 // static 2 KiB per-RHS reduction buffer.
 constexpr int kMaxDynLds = 65536 - 2048;
@@ -78,6 +104,8 @@ struct Tile {
                    // bit 3: XCD-cyclic order over half y-streamed blocks (xcd_cycle_gran)
   int occ = 0;     // cap on resident workgroups per CU (LDS padding); 0 = no cap
   int ystream = 0; // 1: stream each (tblock, zblock) block along y
+  int hcube = 0;   // 1: hypercube work mapping (2x2x2x2 blocks, one per 8 P lanes; tblock / zblock
+                   // then count blocks)
 };
 
 struct DslashCall {

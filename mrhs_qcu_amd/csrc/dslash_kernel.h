@@ -93,6 +93,12 @@ struct DslashArgs {
   uint32_t efac_off;  // > 0 (LDS kernels): byte offset in dynamic LDS of the staged per-RHS epilogue
                       // factors [4][k] in the accumulation type (xb sc_in, xa sc_x, 1 / sc_out, rx_alpha)
   uint32_t xcd_gran;  // > 0: XCD-cyclic workgroup order in chunks of xcd_gran workgroups (xcd_cyclic)
+  // Hypercube kernels (dslash_hcube.h): the launch walks a lattice of
+  // 2x2x2x2 blocks (subLx x Ly/2 x Lz/2 x Lt/2) in the traversal order above,
+  // with tblock / zblock counted in blocks; 2^hc_lp lanes per site.
+  uint32_t hc_lp;
+  int bLy, bLz;
+  FastDiv hdiv_Ly, hdiv_row, hdiv_nzb;
   // Pointer layouts only. Kept last: the packed kernels read only the fields
   // above, which then share a few scalar-cache lines of the argument block
   // (at 24^3x48, 16 RHS, FP32 the packed kernel took 4 us longer with these
@@ -852,6 +858,146 @@ __device__ __forceinline__ void store_nt(V *p, const V &v)
   }
 }
 
+// Epilogue of the packed layout for one lane (R right-hand sides of output
+// site idx): true out = xa * x + xb * sc_in * acc, stored = true / sc_out,
+// optional per-RHS |true out|^2 into nrm; or the fused CG update
+// r -= alpha * true out with |r|^2 into nrm. ef: the per-RHS factors staged
+// in LDS (null: read from global memory).
+template <int N, int R, class CS, class CO, class A>
+__device__ __forceinline__ void epilogue_packed(const DslashArgs &a, const cpx<A> (&acc)[R][4][N], int idx,
+                                                uint32_t slot, const A *ef, A (&nrm)[R])
+{
+  A fin[R], fx[R], fo[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int rhs = int(slot) * R + r;
+    if (ef) {
+      fin[r] = ef[rhs];
+      fx[r] = ef[a.k + rhs];
+      fo[r] = ef[2 * a.k + rhs];
+    } else {
+      fin[r] = A(a.xb) * (a.sc_in ? A(a.sc_in[rhs]) : A(1));
+      fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
+      fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
+    }
+  }
+  if constexpr ((std::is_same<CO, c32>::value && std::is_same<A, float>::value) ||
+                (std::is_same<CO, c64>::value && std::is_same<A, double>::value)) {
+    if (a.rx_r) {
+      const size_t base = size_t(idx) * 4 * N * a.k + size_t(slot) * R;
+      const size_t stride = size_t(a.k) / R;
+      CVec<CO, R> *pr = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_r) + base);
+      const CVec<CS, R> *px =
+          a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
+      A al[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) al[r] = ef ? ef[3 * a.k + int(slot) * R + r] : A(a.rx_alpha[int(slot) * R + r]);
+#pragma unroll
+      for (int sc = 0; sc < 4 * N; ++sc) {
+        CVec<CS, R> xv;
+        if (px) xv = px[sc * stride];
+        CVec<CO, R> rv = pr[sc * stride];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          cpx<A> v = acc[r][sc / N][sc % N];
+          v.re *= fin[r];
+          v.im *= fin[r];
+          if (px) {
+            const cpx<A> xx = ld_cvt<A>(xv.v[r]);
+            v.re += fx[r] * xx.re;
+            v.im += fx[r] * xx.im;
+          }
+          rv.v[r].x -= al[r] * v.re;
+          rv.v[r].y -= al[r] * v.im;
+          nrm[r] += rv.v[r].x * rv.v[r].x + rv.v[r].y * rv.v[r].y;
+        }
+        pr[sc * stride] = rv;
+      }
+      return;
+    }
+  }
+  const size_t base = size_t(idx) * 4 * N * a.k + size_t(slot) * R;
+  const size_t stride = size_t(a.k) / R;
+  CVec<CO, R> *po = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.out) + base);
+  const CVec<CS, R> *px = a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
+  if (a.plain_out) {
+    // The epilogue runs after the wave's last load has returned, so its
+    // instructions are not hidden behind memory: store acc directly.
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) {
+      CVec<CO, R> o;
+#pragma unroll
+      for (int r = 0; r < R; ++r) o.v[r] = StCvt<CO>::go(acc[r][sc / N][sc % N]);
+      if (a.nt_store) store_nt(po + sc * stride, o);
+      else po[sc * stride] = o;
+    }
+  } else if (!px) {
+    // Scaled output without an xpay operand: |fin acc|^2 = fin^2 |acc|^2
+    // and stored = acc (fin / sc_out), two multiplies fewer per value.
+    A gs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) gs[r] = fin[r] * fo[r];
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) {
+      CVec<CO, R> o;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const cpx<A> v = acc[r][sc / N][sc % N];
+        nrm[r] = fmaA(v.re, v.re, fmaA(v.im, v.im, nrm[r]));
+        o.v[r] = StCvt<CO>::go(cpx<A>{v.re * gs[r], v.im * gs[r]});
+      }
+      if (a.nt_store) store_nt(po + sc * stride, o);
+      else po[sc * stride] = o;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) nrm[r] *= fin[r] * fin[r];
+  } else {
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) {
+      CVec<CO, R> o;
+      CVec<CS, R> xv;
+      if (px) xv = px[sc * stride];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cpx<A> v = acc[r][sc / N][sc % N];
+        v.re *= fin[r];
+        v.im *= fin[r];
+        if (px) {
+          const cpx<A> xx = ld_cvt<A>(xv.v[r]);
+          v.re += fx[r] * xx.re;
+          v.im += fx[r] * xx.im;
+        }
+        nrm[r] += v.re * v.re + v.im * v.im;
+        v.re *= fo[r];
+        v.im *= fo[r];
+        o.v[r] = StCvt<CO>::go(v);
+      }
+      if (a.nt_store) store_nt(po + sc * stride, o);
+      else po[sc * stride] = o;
+    }
+  }
+}
+
+// Per-RHS partial sums of |out|^2 of one workgroup (logical index blk) into
+// a.partial[blk][k] (no atomics on global memory: a second pass reduces the
+// [nblocks][k] partials). Every lane of the workgroup calls it.
+template <int R, class A>
+__device__ __forceinline__ void workgroup_partials(const DslashArgs &a, uint32_t blk, bool active, uint32_t slot,
+                                                   const A (&nrm)[R])
+{
+  if (!a.partial) return;
+  __shared__ double red[256];
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.k; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) atomicAdd(&red[int(slot) * R + r], double(nrm[r]));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.k; i += blockDim.x) a.partial[size_t(blk) * a.k + i] = red[i];
+}
+
 // One kernel body for every layout / precision / tile combination.
 //   CS: stored input (and xpay operand) complex, CO: stored output complex,
 //   CG: stored link complex, A: accumulation real type.
@@ -983,122 +1129,14 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
     }
 
     // Epilogue: true out = xa * x + xb * sc_in * acc; stored = true / sc_out.
-    A fin[R], fx[R], fo[R];
     const A *ef = nullptr;
     if constexpr (site_blocked(GL) && LDS)
       if (a.efac_off) ef = reinterpret_cast<const A *>(smem + a.efac_off);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int rhs = int(slot) * R + r;
-      if (ef) {
-        fin[r] = ef[rhs];
-        fx[r] = ef[a.k + rhs];
-        fo[r] = ef[2 * a.k + rhs];
-      } else {
-        fin[r] = A(a.xb) * (a.sc_in ? A(a.sc_in[rhs]) : A(1));
-        fx[r] = A(a.xa) * (a.sc_x ? A(a.sc_x[rhs]) : A(1));
-        fo[r] = a.sc_out ? A(1) / A(a.sc_out[rhs]) : A(1);
-      }
-    }
-    bool fused = false;
-    if constexpr (SL == SL_PACKED && ((std::is_same<CO, c32>::value && std::is_same<A, float>::value) ||
-                                      (std::is_same<CO, c64>::value && std::is_same<A, double>::value))) {
-      if (a.rx_r) {
-        fused = true;
-        const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
-        const size_t stride = size_t(a.k) / R;
-        CVec<CO, R> *pr = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.rx_r) + base);
-        const CVec<CS, R> *px =
-            a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
-        A al[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) al[r] = ef ? ef[3 * a.k + int(slot) * R + r] : A(a.rx_alpha[int(slot) * R + r]);
-#pragma unroll
-        for (int sc = 0; sc < 4 * N; ++sc) {
-          CVec<CS, R> xv;
-          if (px) xv = px[sc * stride];
-          CVec<CO, R> rv = pr[sc * stride];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            cpx<A> v = acc[r][sc / N][sc % N];
-            v.re *= fin[r];
-            v.im *= fin[r];
-            if (px) {
-              const cpx<A> xx = ld_cvt<A>(xv.v[r]);
-              v.re += fx[r] * xx.re;
-              v.im += fx[r] * xx.im;
-            }
-            rv.v[r].x -= al[r] * v.re;
-            rv.v[r].y -= al[r] * v.im;
-            nrm[r] += rv.v[r].x * rv.v[r].x + rv.v[r].y * rv.v[r].y;
-          }
-          pr[sc * stride] = rv;
-        }
-      }
-    }
-    if (fused) {
-    } else if constexpr (SL == SL_PACKED) {
-      const size_t base = size_t(g.idx) * 4 * N * a.k + size_t(slot) * R;
-      const size_t stride = size_t(a.k) / R;
-      CVec<CO, R> *po = reinterpret_cast<CVec<CO, R> *>(static_cast<CO *>(a.out) + base);
-      const CVec<CS, R> *px = a.x ? reinterpret_cast<const CVec<CS, R> *>(static_cast<const CS *>(a.x) + base) : nullptr;
-      if (a.plain_out) {
-        // The epilogue runs after the wave's last load has returned, so its
-        // instructions are not hidden behind memory: store acc directly.
-#pragma unroll
-        for (int sc = 0; sc < 4 * N; ++sc) {
-          CVec<CO, R> o;
-#pragma unroll
-          for (int r = 0; r < R; ++r) o.v[r] = StCvt<CO>::go(acc[r][sc / N][sc % N]);
-          if (a.nt_store) store_nt(po + sc * stride, o);
-          else po[sc * stride] = o;
-        }
-      } else if (!px) {
-        // Scaled output without an xpay operand: |fin acc|^2 = fin^2 |acc|^2
-        // and stored = acc (fin / sc_out), two multiplies fewer per value.
-        A gs[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) gs[r] = fin[r] * fo[r];
-#pragma unroll
-        for (int sc = 0; sc < 4 * N; ++sc) {
-          CVec<CO, R> o;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const cpx<A> v = acc[r][sc / N][sc % N];
-            nrm[r] = fmaA(v.re, v.re, fmaA(v.im, v.im, nrm[r]));
-            o.v[r] = StCvt<CO>::go(cpx<A>{v.re * gs[r], v.im * gs[r]});
-          }
-          if (a.nt_store) store_nt(po + sc * stride, o);
-          else po[sc * stride] = o;
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) nrm[r] *= fin[r] * fin[r];
-      } else {
-#pragma unroll
-        for (int sc = 0; sc < 4 * N; ++sc) {
-          CVec<CO, R> o;
-          CVec<CS, R> xv;
-          if (px) xv = px[sc * stride];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            cpx<A> v = acc[r][sc / N][sc % N];
-            v.re *= fin[r];
-            v.im *= fin[r];
-            if (px) {
-              const cpx<A> xx = ld_cvt<A>(xv.v[r]);
-              v.re += fx[r] * xx.re;
-              v.im += fx[r] * xx.im;
-            }
-            nrm[r] += v.re * v.re + v.im * v.im;
-            v.re *= fo[r];
-            v.im *= fo[r];
-            o.v[r] = StCvt<CO>::go(v);
-          }
-          if (a.nt_store) store_nt(po + sc * stride, o);
-          else po[sc * stride] = o;
-        }
-      }
+    if constexpr (SL == SL_PACKED) {
+      epilogue_packed<N, R, CS, CO, A>(a, acc, g.idx, slot, ef, nrm);
     } else {
+      const A fin[1] = {A(a.xb) * (a.sc_in ? A(a.sc_in[slot]) : A(1))};
+      const A fx[1] = {A(a.xa) * (a.sc_x ? A(a.sc_x[slot]) : A(1))};
       constexpr int C = 16 / int(sizeof(CO));
       bool vec_done = false;
       if constexpr (SL == SL_AOS_PTRS && C > 1) {
@@ -1151,20 +1189,7 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
     }
   }
 
-  if (a.partial) {
-    // Per-RHS partial sums of |out|^2 for this workgroup (no atomics on
-    // global memory: a second pass reduces the [nblocks][k] partials).
-    __shared__ double red[256];
-    __syncthreads();
-    for (int i = threadIdx.x; i < a.k; i += blockDim.x) red[i] = 0.0;
-    __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) atomicAdd(&red[int(slot) * R + r], double(nrm[r]));
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < a.k; i += blockDim.x) a.partial[size_t(blk) * a.k + i] = red[i];
-  }
+  workgroup_partials<R, A>(a, blk, active, slot, nrm);
 }
 
 // The kernel. KF > 0: the RHS count is the compile-time constant KF (packed

@@ -1,0 +1,33 @@
+// Hypercube-mapped dslash instantiations (dslash_hcube.h): packed layout,
+// native links staged in LDS, k / R lanes per site in {1, 2, 4, 8}.
+#include "inst_common.h"
+using namespace qmrhs;
+#define QM_HC(N, R, CS, CO, A, KF) static ::qmrhs::RegHc<N, R, CS, CS, CO, A, KF> QM_CAT(qm_reg_, __COUNTER__);
+QM_HC(3, 2, c32, c32, float, 16)
+QM_HC(2, 1, c32, c32, float, 0)
+QM_HC(2, 2, c32, c32, float, 0)
+QM_HC(3, 1, c32, c32, float, 0)
+QM_HC(3, 2, c32, c32, float, 0)
+QM_HC(4, 1, c32, c32, float, 0)
+QM_HC(4, 2, c32, c32, float, 0)
+QM_HC(3, 1, c64, c64, double, 0)
+// 16 right-hand sides: FP32 (two per lane, and one per lane over two
+// wavefronts), FP64 (one per lane over two wavefronts), FP16 / BF16 and the
+// solver's 16-bit-in FP32-out operator.
+QM_HC(3, 1, c32, c32, float, 16)
+QM_HC(3, 1, c64, c64, double, 16)
+QM_HC(3, 1, c16, c16, float, 16)
+QM_HC(3, 2, c16, c16, float, 16)
+QM_HC(3, 4, c16, c16, float, 16)
+QM_HC(3, 1, cb16, cb16, float, 16)
+QM_HC(3, 2, cb16, cb16, float, 16)
+QM_HC(3, 1, c16, c32, float, 16)
+QM_HC(3, 2, c16, c32, float, 16)
+QM_HC(3, 4, c16, c32, float, 16)
+// Run-time RHS count, 16-bit storage.
+QM_HC(3, 1, c16, c16, float, 0)
+QM_HC(3, 2, c16, c16, float, 0)
+QM_HC(3, 1, cb16, cb16, float, 0)
+QM_HC(3, 2, cb16, cb16, float, 0)
+QM_HC(3, 1, cb16, c32, float, 16)
+QM_HC(3, 2, cb16, c32, float, 16)

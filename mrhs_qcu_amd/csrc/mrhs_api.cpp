@@ -128,7 +128,8 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.zblock = v[12];
     t.hints = v[13];
     t.occ = v[14];
-    t.ystream = v[15];
+    t.ystream = v[15] & 1;
+    t.hcube = (v[15] >> 1) & 1;
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -142,7 +143,7 @@ void append_tune_file(const TuneKey &k, const Tile &t)
   if (!f) return;
   fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), std::get<0>(k),
           std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k),
-          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream);
+          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream | (t.hcube << 1));
   fclose(f);
 }
 
@@ -261,6 +262,39 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
       }
     }
   }
+  // Hypercube tiles (one wavefront per 2x2x2x2 block of output sites):
+  // y-streamed blocks of the block lattice, both XCD orders.
+  for (int R : {1, 2, 4}) {
+    const int P = k % R ? 0 : k / R;
+    if (!P || P > 32 || (P & (P - 1))) continue;
+    KernelKey key{SL_PACKED, GL_NATIVE, g.N, R, 1, 0, prec, gauge_prec_for(prec), prec};
+    key.hc = 1;
+    KernelKey key_kf = key;
+    key_kf.kf = k;
+    if (!find_kernel(key) && !find_kernel(key_kf)) continue;
+    for (int block : {128, 256}) {
+      if (block % (8 * P)) continue;
+      const size_t shm = size_t(block / P) * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
+      if (shm > size_t(kMaxDynLds)) continue;
+      for (int tb : {1, 2, 3, 4, 6, 8}) {
+        if ((g.L[3] / 2) % tb || tb >= g.L[3] / 2) continue;
+        for (int zb : {1, 2, 3, 4, 6}) {
+          if ((g.L[2] / 2) % zb || zb >= g.L[2] / 2) continue;
+          Tile t;
+          t.R = R;
+          t.lds = 1;
+          t.block = block;
+          t.tblock = tb;
+          t.zblock = zb;
+          t.ystream = 1;
+          t.hcube = 1;
+          out.push_back(t);
+          t.hints = 8;
+          if (xcd_cycle_gran(g, t, k)) out.push_back(t);
+        }
+      }
+    }
+  }
   return out;
 }
 
@@ -274,7 +308,8 @@ Tile tile_from_c(const qcuMrhsTile &c)
   t.zblock = c.z_block;
   t.hints = c.cache_hints;
   t.occ = c.occupancy;
-  t.ystream = c.traversal;
+  t.ystream = c.traversal & 1;
+  t.hcube = (c.traversal >> 1) & 1;
   return t;
 }
 
@@ -288,7 +323,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
   c.occupancy = t.occ;
-  c.traversal = t.ystream;
+  c.traversal = t.ystream | (t.hcube << 1);
   return c;
 }
 
@@ -562,6 +597,20 @@ int qcuMrhsGetTile(const qcuMrhsGeom *gc, int prec, int nrhs, qcuMrhsTile *tile)
   return 0;
 }
 
+int qcuMrhsSetTile(const qcuMrhsGeom *gc, int prec, int nrhs, const qcuMrhsTile *tile)
+{
+  Geom g;
+  std::string err;
+  if (!geom_from_c(gc, &g, &err)) return api_fail(err);
+  if (!tile) return api_fail("null tile");
+  if (nrhs < 1) return api_fail("nrhs must be >= 1");
+  const Tile t = tile_from_c(*tile);
+  if (t.R != 1 && t.R != 2 && t.R != 4) return api_fail("rhs_per_lane must be 1, 2 or 4");
+  if (t.block < 64 || t.block > 256 || t.block % 64) return api_fail("block must be 64, 128, 192 or 256");
+  set_tile(g, prec, nrhs, t);
+  return 0;
+}
+
 int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs, const qcuMrhsGeom *gc,
                 qcuMrhsTile *best, double *best_us, void *stream)
 {
@@ -595,7 +644,7 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   auto log_tile = [&](const char *round, const Tile &t, double us) {
     if (log)
       fprintf(stderr, "qcuMrhsTune %s N=%d k=%d prec=%d tile=%d,%d,%d,%d,%d,%d,%d,%d us=%.2f\n", round, g.N, nrhs,
-              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream, us);
+              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream | (t.hcube << 1), us);
   };
   for (const Tile &t : cands) {
     const double us = time_tile(t, 5);

@@ -289,7 +289,33 @@ def test_pinned_tile_table():
         N, k, prec, Lx, Ly, Lz, Lt, mask, R, block, lds, tb, zb, hints, occ, trav = map(int, f[1:])
         assert k % R == 0 and block in (64, 128, 192, 256) and lds in (0, 1), line
         assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
+        assert trav in (0, 1, 2, 3), line
+        if trav & 2:
+            # hypercube mapping: blocking counted in 2x2x2x2 blocks, whole
+            # blocks of 8 P lanes per workgroup, links staged in LDS
+            P = k // R
+            assert P & (P - 1) == 0 and P <= 32 and block % (8 * P) == 0 and lds == 1 and N <= 4, line
+            assert (Lt // 2) % max(tb, 1) == 0 and (zb == 0 or (Lz // 2) % zb == 0), line
         shapes.add((N, k, prec, (Lx, Ly, Lz, Lt), mask))
     for L in ((24, 24, 24, 48), (32, 32, 32, 64)):
         for prec in (0, 1, 2, 4, 5):
             assert (3, 16, prec, L, 0) in shapes, (L, prec)
+
+
+def test_set_tile_round_trip(lib):
+    """qcuMrhsSetTile pins a shape's tile for this process (hypercube tiles
+    included, traversal bit 1); qcuMrhsGetTile returns it; malformed tiles are
+    refused with a message."""
+    from mrhs_qcu_amd import _lib
+
+    g = _lib.geom((6, 4, 10, 14), 3)
+    want = (2, 128, 1, 7, 5, 8, 0, 3)
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(*want))) == 0
+    t = _lib.Tile()
+    assert lib.qcuMrhsGetTile(g, _lib.SINGLE, 8, ctypes.byref(t)) == 0
+    assert t.as_tuple() == want
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(3, 128))) != 0
+    assert b"rhs_per_lane" in lib.qcuMrhsLastError()
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 96))) != 0
+    assert b"block" in lib.qcuMrhsLastError()
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, None) != 0

@@ -1,0 +1,250 @@
+"""GPU parity of the hypercube-mapped dslash (dslash_hcube.h) against the
+FP64 oracle: every hypercube tile shape (RHS per lane, workgroup size,
+block-lattice blocking, both traversals and XCD orders), N = 2..4, both
+parities and dagger signs, split-lattice boundary skips, the xpay epilogue;
+FP32 within 1e-6 and FP64 within 1e-13 (per-RHS relative error, max over RHS).
+"""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from mrhs_qcu_amd import _lib, fields
+from oracle import oracle
+from helpers import rel_err
+from test_dslash_gpu import pack_inputs, unpack, stream, TOL
+
+pytestmark = pytest.mark.gpu
+
+HC = 2  # traversal bit of the hypercube mapping
+
+
+def hc_tiles(L, k, blocks=(64, 128, 192, 256)):
+    bt, bz = L[3] // 2, L[2] // 2
+    for R in (1, 2, 4):
+        if k % R or k // R > 8 or (k // R) & (k // R - 1):
+            continue
+        for block in blocks:
+            for tb in [d for d in (1, 2, 3, 4) if bt % d == 0]:
+                for zb in [d for d in (0, 1, 2, 3) if d == 0 or bz % d == 0]:
+                    for ys, hints in ((0, 0), (1, 0), (1, 8)):
+                        yield (R, block, 1, tb, zb, hints, 0, HC | ys)
+
+
+def test_hcube_every_tile_fp32(lib, dev):
+    """Every hypercube tile for N = 3, k = 8, 4, 2, 1 against the oracle."""
+    L, N = (8, 4, 6, 8), 3
+    g = fields.gauge_field(L, N, seed=21)
+    for k in (8, 4, 2, 1):
+        psi = fields.gaussian_spinors(L, N, k, seed=22 + k)
+        want = {pd: oracle.dslash(g, psi, L, *pd) for pd in ((0, 0), (1, 1))}
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+        n = 0
+        for i, t in enumerate(hc_tiles(L, k)):
+            parity, dagger = ((0, 0), (1, 1))[i & 1]
+            pout = torch.zeros_like(pin)
+            rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, parity, dagger,
+                                   geom, _lib.tile(*t), stream())
+            if rc != 0:
+                msg = lib.qcuMrhsLastError().decode()
+                assert "no kernel instantiated" in msg or "LDS" in msg, (t, msg)
+                continue
+            got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+            err = rel_err(got, want[(parity, dagger)])
+            assert err < TOL[1], (k, t, err)
+            n += 1
+        assert n >= 20, (k, n)
+
+
+@pytest.mark.parametrize("N,L,k", [(2, (4, 4, 4, 8), 4), (4, (4, 6, 4, 4), 2), (3, (2, 2, 2, 2), 2),
+                                   (3, (12, 2, 4, 6), 8)])
+def test_hcube_colours_and_small_extents(lib, dev, N, L, k):
+    """N = 2..4 and extents down to 2 (neighbours that wrap onto the block)."""
+    g = fields.gauge_field(L, N, seed=31 + N)
+    psi = fields.gaussian_spinors(L, N, k, seed=32 + N)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+    n = 0
+    for t in hc_tiles(L, k, blocks=(64, 256)):
+        for parity, dagger in ((0, 1), (1, 0)):
+            pout = torch.zeros_like(pin)
+            rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, parity, dagger,
+                                   geom, _lib.tile(*t), stream())
+            if rc != 0:
+                msg = lib.qcuMrhsLastError().decode()
+                assert "no kernel instantiated" in msg or "LDS" in msg, (t, msg)
+                continue
+            got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger)) < TOL[1], (t, parity, dagger)
+            n += 1
+    assert n > 0
+
+
+def test_hcube_boundary_skip(lib, dev):
+    """Split-lattice semantics (terms crossing a split dimension skipped)."""
+    L, N, k = (8, 4, 4, 8), 3, 8
+    g = fields.gauge_field(L, N, seed=41)
+    psi = fields.gaussian_spinors(L, N, k, seed=42)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+    for grid in ((2, 1, 1, 1), (1, 2, 2, 1), (1, 1, 1, 4), (2, 2, 2, 2)):
+        for d in range(4):
+            geom.grid[d] = grid[d]
+        for parity in (0, 1):
+            pout = torch.zeros_like(pin)
+            _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, parity, 0,
+                                         geom, _lib.tile(2, 128, 1, 2, 1, 0, 0, HC | 1), stream()))
+            got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+            want = oracle.dslash(g, psi, L, parity, 0, grid=grid)
+            assert rel_err(got, want) < TOL[1], (grid, parity)
+
+
+def test_hcube_fp64(lib, dev):
+    """FP64 (one RHS per lane) within 1e-13 of the oracle."""
+    L, N = (4, 4, 6, 8), 3
+    g = fields.gauge_field(L, N, seed=51)
+    for k in (2, 4, 8):
+        psi = fields.gaussian_spinors(L, N, k, seed=52 + k)
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.DOUBLE)
+        for parity, dagger in ((0, 0), (1, 1)):
+            pout = torch.zeros_like(pin)
+            _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.DOUBLE, k, parity,
+                                         dagger, geom, _lib.tile(1, 64, 1, 2, 3, 0, 0, HC | 1), stream()))
+            got = unpack(lib, dev, pout, _lib.DOUBLE, k, geom, psi.shape[1:])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger)) < TOL[2], (k, parity, dagger)
+
+
+def test_hcube_xpay_and_k16(lib, dev):
+    """out = a x + b D in through the hypercube kernel compiled for 16 RHS."""
+    L, N, k = (8, 8, 4, 8), 3, 16
+    g = fields.gauge_field(L, N, seed=61)
+    psi = fields.gaussian_spinors(L, N, k, seed=62)
+    xs = fields.gaussian_spinors(L, N, k, seed=63)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+    _, _, px = pack_inputs(lib, dev, g, xs, L, N, _lib.SINGLE)
+    a, b = 0.75, -0.3
+    for parity, dagger in ((0, 0), (1, 1)):
+        pout = torch.zeros_like(pin)
+        _lib.check(lib.qcuMrhsDslashXpay(pout.data_ptr(), pin.data_ptr(), px.data_ptr(), a, b, gnat.data_ptr(),
+                                         _lib.SINGLE, k, parity, dagger, geom, _lib.tile(2, 128, 1, 2, 2, 8, 0, HC | 1),
+                                         stream()))
+        got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+        want = a * xs + b * oracle.dslash(g, psi, L, parity, dagger)
+        assert rel_err(got, want) < TOL[1], (parity, dagger)
+
+
+def test_hcube_rejects_bad_tiles(lib, dev):
+    """Shapes without a hypercube kernel are refused with a message."""
+    L, N, k = (8, 4, 6, 8), 3, 16
+    g = fields.gauge_field(L, N, seed=71)
+    psi = fields.gaussian_spinors(L, N, k, seed=72)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+    pout = torch.zeros_like(pin)
+    for t, msg in (((1, 64, 1, 1, 0, 0, 0, HC), "multiple of 8 * nrhs"),     # 16 lanes per site
+                   ((2, 128, 1, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
+                   ((2, 128, 0, 3, 0, 0, 0, HC), "t_block must divide Lt / 2")):
+        rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 0, 0, geom,
+                               _lib.tile(*t), stream())
+        assert rc != 0 and msg in lib.qcuMrhsLastError().decode(), (t, lib.qcuMrhsLastError())
+
+
+def test_hcube_tile_without_kernel_runs_site_major(lib, dev):
+    """A hypercube tile for a shape without a hypercube kernel (links not
+    staged, N = 5, a 16-bit mixed output with run-time k) runs the site-major
+    kernel of the same blocking and gives the oracle's result."""
+    for N, L, k, t in ((3, (8, 4, 6, 8), 8, (2, 128, 0, 2, 3, 0, 0, HC | 1)),
+                       (5, (4, 4, 4, 8), 4, (1, 128, 1, 2, 1, 8, 0, HC | 1))):
+        g = fields.gauge_field(L, N, seed=81 + N)
+        psi = fields.gaussian_spinors(L, N, k, seed=82 + N)
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+        pout = torch.zeros_like(pin)
+        _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 1, 0, geom,
+                                     _lib.tile(*t), stream()))
+        got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+        assert rel_err(got, oracle.dslash(g, psi, L, 1, 0)) < TOL[1], (N, t)
+
+
+@pytest.mark.parametrize("prec", [_lib.HALF, _lib.BF16])
+def test_hcube_16bit(lib, dev, prec):
+    """FP16 / BF16 storage through the hypercube kernels (k = 16 compiled and
+    run-time k), within the stated storage tolerance of the FP64 oracle."""
+    L, N = (8, 4, 6, 8), 3
+    g = fields.gauge_field(L, N, seed=91, smear_eps=0.3)
+    for k, Rs in ((16, (1, 2, 4)), (8, (1, 2))):
+        psi = fields.gaussian_spinors(L, N, k, seed=92 + k)
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+        n = 0
+        for R in Rs:
+            for block in (128, 256):
+                if block % (8 * (k // R)):
+                    continue
+                for parity, dagger in ((0, 0), (1, 1)):
+                    t = (R, block, 1, 2, 3, 8 if parity else 0, 0, HC | 1)
+                    pout = torch.zeros_like(pin)
+                    rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger,
+                                           geom, _lib.tile(*t), stream())
+                    if rc != 0:
+                        msg = lib.qcuMrhsLastError().decode()
+                        assert "LDS" in msg, (t, msg)
+                        continue
+                    got = unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+                    err = rel_err(got, oracle.dslash(g, psi, L, parity, dagger))
+                    print(f"prec={prec} k={k} tile={t}: {err:.2e}")
+                    assert err < TOL[prec], (k, t, err)
+                    n += 1
+        assert n >= 4, (k, n)
+
+
+@pytest.mark.parametrize("prec", [_lib.SINGLE, _lib.DOUBLE])
+def test_hcube_rhs_split_over_two_wavefronts(lib, dev, prec):
+    """One RHS per lane at k = 16 (16 lanes per site: each block's slots split
+    over two wavefronts), FP32 and FP64, both workgroup sizes."""
+    L, N, k = (8, 4, 6, 8), 3, 16
+    g = fields.gauge_field(L, N, seed=95)
+    psi = fields.gaussian_spinors(L, N, k, seed=96)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+    for block in (128, 256):
+        for parity, dagger, ys, hints in ((0, 0, 1, 8), (1, 1, 0, 0)):
+            pout = torch.zeros_like(pin)
+            rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger, geom,
+                                   _lib.tile(1, block, 1, 2, 1, hints, 0, HC | ys), stream())
+            if rc != 0:
+                assert "LDS" in lib.qcuMrhsLastError().decode()
+                continue
+            got = unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger)) < TOL[prec], (block, parity)
+
+
+def test_hcube_cg(lib, dev):
+    """The mixed FP16 / FP64 CG with hypercube tiles on every operator it
+    launches (16-bit in, FP32 out, scales, norms, the fused r update, FP32 and
+    FP64 reliable updates): true residual recomputed by the oracle <= 2e-10,
+    mixed solution within 1e-8 of the FP64 one."""
+    from test_cg_gpu import solve, check_solution
+
+    L, N, k, kappa = (8, 8, 4, 8), 3, 16, 0.13
+    geom = _lib.geom(L, N)
+    tiles = {_lib.HALF: (2, 128, 1, 2, 1, 0, 0, HC | 1), _lib.SINGLE: (2, 128, 1, 1, 2, 0, 0, HC | 1),
+             _lib.DOUBLE: (1, 128, 1, 2, 1, 0, 0, HC | 1)}
+    old = {}
+    for prec, t in tiles.items():
+        o = _lib.Tile()
+        _lib.check(lib.qcuMrhsGetTile(geom, prec, k, ctypes.byref(o)))
+        old[prec] = o
+        _lib.check(lib.qcuMrhsSetTile(geom, prec, k, ctypes.byref(_lib.tile(*t))))
+    try:
+        g = fields.gauge_field(L, N, seed=97)
+        b = fields.cg_sources(L, N, k, seed=98, n_point=6)
+        x64, p64, r64 = solve(lib, dev, g, b, L, N, kappa, _lib.DOUBLE)
+        x16, p16, r16 = solve(lib, dev, g, b, L, N, kappa, _lib.HALF)
+        print(f"hypercube tiles: fp64 {p64.iterations} it, mixed {p16.iterations} it / {p16.reliable_updates} updates")
+        assert max(r64) <= 1e-10 and max(r16) <= 1e-10
+        assert check_solution(g, b, x16, L, kappa, 1e-10) <= 2e-10
+        assert check_solution(g, b, x64, L, kappa, 1e-10) <= 2e-10
+        for i in range(k):
+            d = np.linalg.norm(x16[i] - x64[i]) / np.linalg.norm(x64[i])
+            assert d <= 1e-8, (i, d)
+    finally:
+        for prec, o in old.items():
+            lib.qcuMrhsSetTile(geom, prec, k, ctypes.byref(o))
+        lib.qcuMrhsSolverRelease()
