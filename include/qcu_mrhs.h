@@ -226,7 +226,12 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
 int qcuMrhsGetTile(const qcuMrhsGeom *g, int prec, int nrhs, qcuMrhsTile *tile);
 /* Sets the tile the dispatcher, qcuMrhsDslashEx and the solver use for a
  * shape in this process (also appended to the QCU_MRHS_TUNE_CACHE file when
- * that is set). The tile is checked when it is next launched. */
+ * that is set). The tile is checked when it is next launched.
+ * GetTile / SetTile with prec | QCU_MRHS_TILE_USER_LAYOUT name the tile of
+ * the user-layout path (qcu.h startDslash, qcuMrhsDslashUser); only its
+ * hypercube tiles (traversal bit 1, block a multiple of 8 nrhs) change that
+ * path, which otherwise keeps its built-in traversal. */
+#define QCU_MRHS_TILE_USER_LAYOUT 8
 int qcuMrhsSetTile(const qcuMrhsGeom *g, int prec, int nrhs, const qcuMrhsTile *tile);
 /* Number of instantiated dslash kernels (for diagnostics). */
 int qcuMrhsKernelCount(void);

@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqcu.so")
 
 HALF, SINGLE, DOUBLE, BF16, FP8 = 0, 1, 2, 4, 5
+TILE_USER_LAYOUT = 8  # precision flag of GetTile / SetTile: the user-layout (AoS pointer) path
 BYTES = {HALF: 4, SINGLE: 8, DOUBLE: 16, BF16: 4, FP8: 2}  # per complex (FP8 links are stored as HALF)
 
 

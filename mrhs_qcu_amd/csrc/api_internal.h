@@ -17,6 +17,10 @@ int prec_bytes(int prec);
 long long sites_per_parity(const Geom &g);
 
 Tile tile_for(const Geom &g, int prec, int k);
+// Tile of the user-layout (AoS pointer) path: the tile table's entry for
+// precision code prec | QCU_MRHS_TILE_USER_LAYOUT, else a tile without the
+// hypercube bit (the pointer launch then keeps its own traversal).
+Tile user_layout_tile(const Geom &g, int prec, int k);
 Tile tile_from_c(const qcuMrhsTile &c);
 qcuMrhsTile tile_to_c(const Tile &t);
 void set_tile(const Geom &g, int prec, int k, const Tile &t);

@@ -312,7 +312,7 @@ static int check_call(const DslashCall &c, std::string *err)
   } else {
     // Counted in 2x2x2x2 blocks (so they also divide Lt and Lz when the
     // site-major kernel runs the tile).
-    if (c.sl != SL_PACKED) return fail(err, "hypercube tiles are for the packed layout"), 1;
+    if (c.sl == SL_SOA_PTRS) return fail(err, "hypercube tiles are for the packed and user AoS layouts"), 1;
     if (c.tile.tblock > 1 && (c.geom.L[3] / 2) % c.tile.tblock)
       return fail(err, "hypercube t_block must divide Lt / 2"), 1;
     if (c.tile.zblock > 0 && (c.geom.L[2] / 2) % c.tile.zblock)
@@ -426,6 +426,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     b.aos_vec = (ptr_bits % 16) == 0 ? 1 : 0;
     size_t shm = 0;
     LaunchFn f = fn;
+    int Bl = B;
     if (fn_lds) {
       const size_t lds_bytes = size_t((B - 1) / kk + 2) * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
       if (lds_bytes <= size_t(kMaxDynLds)) {
@@ -433,7 +434,41 @@ int launch_dslash(const DslashCall &c, std::string *err)
         shm = lds_bytes;
       }
     }
-    hipError_t e = f(b, dim3(b.nblocks), dim3(B), shm, c.stream);
+    // A hypercube tile (the caller's, from the user-layout entry of the tile
+    // table) runs the AoS pointers through the hypercube kernel when this
+    // chunk's RHS count is a power of two, the workgroup holds whole blocks
+    // and the links fit in LDS; otherwise the launch above is kept.
+    if (c.sl == SL_AOS_PTRS && c.tile.hcube && kk <= 32 && !(kk & (kk - 1)) && c.tile.block % (8 * kk) == 0 &&
+        c.tile.block >= 64 && c.tile.block <= 256) {
+      KernelKey kh = key_lds;
+      kh.hc = 1;
+      LaunchFn fh = find_kernel(kh);
+      const int Bh = c.tile.block;
+      const size_t lds_h = size_t(Bh / kk) * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
+      if (fh && lds_h <= size_t(kMaxDynLds)) {
+        const int bz = b.Lz / 2;
+        b.tblock = c.tile.tblock < 1 ? 1 : c.tile.tblock;
+        b.zblock = c.tile.zblock < 1 ? bz : c.tile.zblock;
+        b.ystream = c.tile.ystream ? 1 : 0;
+        b.bLy = b.Ly / 2;
+        b.bLz = bz;
+        b.div_tb = make_fastdiv(uint32_t(b.tblock));
+        b.div_zb = make_fastdiv(uint32_t(b.zblock));
+        b.hdiv_Ly = make_fastdiv(uint32_t(b.bLy));
+        b.hdiv_row = make_fastdiv(uint32_t(b.subLx * b.bLy));
+        b.hdiv_nzb = make_fastdiv(uint32_t(bz / b.zblock));
+        b.hc_lp = uint32_t(__builtin_ctz(unsigned(kk)));
+        b.nblocks = uint32_t((b.nlanes + uint32_t(Bh) - 1) / uint32_t(Bh));
+        b.xcd_map = (c.tile.hints & 4) ? 0 : 1;
+        Tile t = c.tile;
+        t.R = 1;
+        b.xcd_gran = xcd_cycle_gran(c.geom, t, kk);
+        f = fh;
+        shm = lds_h;
+        Bl = Bh;
+      }
+    }
+    hipError_t e = f(b, dim3(b.nblocks), dim3(Bl), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
   }
   return 0;

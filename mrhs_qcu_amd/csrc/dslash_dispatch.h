@@ -55,27 +55,29 @@ struct Reg {
   }
 };
 
-template <int N, int R, class CS, class CG, class CO, class A, int KF>
+template <int N, int R, int SL, class CS, class CG, class CO, class A, int KF>
 hipError_t launch_hc_impl(const DslashArgs &a, dim3 g, dim3 b, size_t shm, hipStream_t st)
 {
-  hipLaunchKernelGGL((dslash_hc_kernel<N, R, CS, CG, CO, A, KF>), g, b, shm, st, a);
+  hipLaunchKernelGGL((dslash_hc_kernel<N, R, SL, CS, CG, CO, A, KF>), g, b, shm, st, a);
   return hipGetLastError();
 }
-template <int N, int R, class CS, class CG, class CO, class A, int KF>
+template <int N, int R, int SL, class CS, class CG, class CO, class A, int KF>
 hipError_t attr_hc_impl(hipFuncAttributes *at)
 {
-  return hipFuncGetAttributes(at, reinterpret_cast<const void *>(&dslash_hc_kernel<N, R, CS, CG, CO, A, KF>));
+  return hipFuncGetAttributes(at, reinterpret_cast<const void *>(&dslash_hc_kernel<N, R, SL, CS, CG, CO, A, KF>));
 }
 
-// Hypercube kernels: packed layout, native links staged in LDS.
-template <int N, int R, class CS, class CG, class CO, class A, int KF = 0>
+// Hypercube kernels: packed layout (or user AoS pointers), native links
+// staged in LDS.
+template <int N, int R, class CS, class CG, class CO, class A, int KF = 0, int SL = SL_PACKED>
 struct RegHc {
   RegHc()
   {
-    KernelKey key{SL_PACKED, GL_NATIVE, N, R, 1, 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec, StoreInfo<CO>::prec};
+    KernelKey key{SL, GL_NATIVE, N, R, 1, 0, StoreInfo<CS>::prec, StoreInfo<CG>::prec, StoreInfo<CO>::prec};
     key.kf = KF;
     key.hc = 1;
-    register_kernel(key, &launch_hc_impl<N, R, CS, CG, CO, A, KF>, &attr_hc_impl<N, R, CS, CG, CO, A, KF>);
+    register_kernel(key, &launch_hc_impl<N, R, SL, CS, CG, CO, A, KF>,
+                    &attr_hc_impl<N, R, SL, CS, CG, CO, A, KF>);
   }
 };
 

@@ -265,12 +265,13 @@ __device__ __forceinline__ void hc_hop_xchg_p(cpx<float> (&acc)[R][4][N], const 
   }
 }
 
-// Kernel body. Packed spinors, native links staged in LDS, N <= 4 (links in
+// Kernel body. Packed spinors (or the user AoS layout through per-RHS
+// pointers, one RHS per lane), native links staged in LDS, N <= 4 (links in
 // registers), P = k / R lanes per site a power of two <= 32 with 8 P dividing
 // the workgroup. The launch has S * P lanes; S is a multiple of 8 (even Ly,
 // Lz, Lt), so every block is whole and the corner partners of an active lane
 // are active.
-template <int N, int R, class CS, class CG, class CO, class A>
+template <int N, int R, int SL, class CS, class CG, class CO, class A>
 __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_blk)
 {
   static_assert(N < kRowLinkN, "hypercube kernels hold whole links in registers");
@@ -356,7 +357,7 @@ __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};
 
-    constexpr bool kPair = std::is_same<CS, CG>::value &&
+    constexpr bool kPair = SL == SL_PACKED && std::is_same<CS, CG>::value &&
                            (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) &&
                            std::is_same<A, float>::value;
     // Inner hops: x from the own spinor, y / z / t through the partners;
@@ -380,39 +381,42 @@ __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_
       if (lt) hc_hop_p<CS, N, 3, R>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
     } else {
       cpx<A> psA[R][4][N], psB[R][4][N];
-      load_spinor<N, R, SL_PACKED, CS, A>(psA, a, a.in, idx, int(slot));  // own inner site
-      load_spinor<N, R, SL_PACKED, CS, A>(psB, a, a.in, nx, int(slot));   // x outer
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, idx, int(slot));  // own inner site
+      load_spinor<N, R, SL, CS, A>(psB, a, a.in, nx, int(slot));   // x outer
       hc_hop<N, 0, R, CG, A>(acc, psA, blk8, e ? 4 : 0, sx);
       hc_hop_xchg<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 5 : 1, sy, int(Pl));
       hc_hop_xchg<N, 2, R, CG, A>(acc, psA, blk8, (cn & 2u) ? 6 : 2, sz, int(2 * Pl));
       hc_hop_xchg<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 7 : 3, stt, int(4 * Pl));
-      load_spinor<N, R, SL_PACKED, CS, A>(psA, a, a.in, ny, int(slot));
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, ny, int(slot));
       if (lx) hc_hop<N, 0, R, CG, A>(acc, psB, blk8, e ? 0 : 4, -sx);
-      load_spinor<N, R, SL_PACKED, CS, A>(psB, a, a.in, nz, int(slot));
+      load_spinor<N, R, SL, CS, A>(psB, a, a.in, nz, int(slot));
       if (ly) hc_hop<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 1 : 5, -sy);
-      load_spinor<N, R, SL_PACKED, CS, A>(psA, a, a.in, nt, int(slot));
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, nt, int(slot));
       if (lz) hc_hop<N, 2, R, CG, A>(acc, psB, blk8, (cn & 2u) ? 2 : 6, -sz);
       if (lt) hc_hop<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
     }
 
-    epilogue_packed<N, R, CS, CO, A>(a, acc, idx, slot, ef, nrm);
+    if constexpr (SL == SL_PACKED) epilogue_packed<N, R, CS, CO, A>(a, acc, idx, slot, ef, nrm);
+    else epilogue_ptrs<N, SL, CS, CO, A>(a, acc, idx, slot, nrm);
   }
   workgroup_partials<R, A>(a, blk, active, slot, nrm);
 }
 
 // KF > 0: compiled for exactly KF right-hand sides (constant row offsets and
 // lane shifts).
-template <int N, int R, class CS, class CG, class CO, class A, int KF = 0>
+template <int N, int R, int SL, class CS, class CG, class CO, class A, int KF = 0>
 __global__ void __launch_bounds__(256) dslash_hc_kernel(const DslashArgs a)
 {
+  static_assert(SL == SL_PACKED || (SL == SL_AOS_PTRS && R == 1 && KF == 0),
+                "hypercube kernels: packed spinors, or user AoS pointers with one RHS per lane");
   if constexpr (KF > 0) {
     static_assert(KF % R == 0 && (KF / R) <= 32 && ((KF / R) & (KF / R - 1)) == 0, "k / R: a power of two <= 32");
     DslashArgs b = a;
     b.k = KF;
     b.hc_lp = uint32_t(__builtin_ctz(unsigned(KF / R)));
-    dslash_hc_body<N, R, CS, CG, CO, A>(b, blockIdx.x);
+    dslash_hc_body<N, R, SL, CS, CG, CO, A>(b, blockIdx.x);
   } else {
-    dslash_hc_body<N, R, CS, CG, CO, A>(a, blockIdx.x);
+    dslash_hc_body<N, R, SL, CS, CG, CO, A>(a, blockIdx.x);
   }
 }
 

@@ -978,6 +978,66 @@ __device__ __forceinline__ void epilogue_packed(const DslashArgs &a, const cpx<A
   }
 }
 
+// Epilogue of the pointer layouts (one RHS per lane, RHS slot `slot`):
+// out = xb sc_in acc (+ xa sc_x x with a per-RHS xpay operand), written to
+// that RHS's buffer at site idx; |out|^2 into nrm.
+template <int N, int SL, class CS, class CO, class A>
+__device__ __forceinline__ void epilogue_ptrs(const DslashArgs &a, const cpx<A> (&acc)[1][4][N], int idx,
+                                              uint32_t slot, A (&nrm)[1])
+{
+  const A fin[1] = {A(a.xb) * (a.sc_in ? A(a.sc_in[slot]) : A(1))};
+  const A fx[1] = {A(a.xa) * (a.sc_x ? A(a.sc_x[slot]) : A(1))};
+  constexpr int C = 16 / int(sizeof(CO));
+  bool vec_done = false;
+  if constexpr (SL == SL_AOS_PTRS && C > 1) {
+    if (a.aos_vec && !a.xptr) {
+      CVec<CO, C> *pv = reinterpret_cast<CVec<CO, C> *>(static_cast<CO *>(a.out_ptrs.p[slot]) +
+                                                        size_t(idx) * 4 * N);
+#pragma unroll
+      for (int q = 0; q < 4 * N / C; ++q) {
+        CVec<CO, C> o;
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+          cpx<A> v = acc[0][(q * C + j) / N][(q * C + j) % N];
+          v.re *= fin[0];
+          v.im *= fin[0];
+          nrm[0] += v.re * v.re + v.im * v.im;
+          o.v[j] = StCvt<CO>::go(v);
+        }
+        pv[q] = o;
+      }
+      vec_done = true;
+    }
+  }
+  if (!vec_done) {
+    CO *po;
+    const CS *px = nullptr;  // xpay operand of this RHS (same layout as the input)
+    size_t es;
+    if constexpr (SL == SL_AOS_PTRS) {
+      po = static_cast<CO *>(a.out_ptrs.p[slot]) + size_t(idx) * 4 * N;
+      if (a.xptr) px = static_cast<const CS *>(a.x_ptrs.p[slot]) + size_t(idx) * 4 * N;
+      es = 1;
+    } else {
+      po = static_cast<CO *>(a.out_ptrs.p[slot]) + idx;
+      if (a.xptr) px = static_cast<const CS *>(a.x_ptrs.p[slot]) + idx;
+      es = size_t(a.S);
+    }
+#pragma unroll
+    for (int sc = 0; sc < 4 * N; ++sc) {
+      cpx<A> v = acc[0][sc / N][sc % N];
+      v.re *= fin[0];
+      v.im *= fin[0];
+      if (px) {
+        const cpx<A> xx = ld_cvt<A>(px[sc * es]);
+        v.re = fmaA(fx[0], xx.re, v.re);
+        v.im = fmaA(fx[0], xx.im, v.im);
+      }
+      nrm[0] += v.re * v.re + v.im * v.im;
+      po[sc * es] = StCvt<CO>::go(v);
+    }
+  }
+}
+
 // Per-RHS partial sums of |out|^2 of one workgroup (logical index blk) into
 // a.partial[blk][k] (no atomics on global memory: a second pass reduces the
 // [nblocks][k] partials). Every lane of the workgroup calls it.
@@ -1135,57 +1195,7 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
     if constexpr (SL == SL_PACKED) {
       epilogue_packed<N, R, CS, CO, A>(a, acc, g.idx, slot, ef, nrm);
     } else {
-      const A fin[1] = {A(a.xb) * (a.sc_in ? A(a.sc_in[slot]) : A(1))};
-      const A fx[1] = {A(a.xa) * (a.sc_x ? A(a.sc_x[slot]) : A(1))};
-      constexpr int C = 16 / int(sizeof(CO));
-      bool vec_done = false;
-      if constexpr (SL == SL_AOS_PTRS && C > 1) {
-        if (a.aos_vec && !a.xptr) {
-          CVec<CO, C> *pv = reinterpret_cast<CVec<CO, C> *>(static_cast<CO *>(a.out_ptrs.p[slot]) +
-                                                            size_t(g.idx) * 4 * N);
-#pragma unroll
-          for (int q = 0; q < 4 * N / C; ++q) {
-            CVec<CO, C> o;
-#pragma unroll
-            for (int j = 0; j < C; ++j) {
-              cpx<A> v = acc[0][(q * C + j) / N][(q * C + j) % N];
-              v.re *= fin[0];
-              v.im *= fin[0];
-              nrm[0] += v.re * v.re + v.im * v.im;
-              o.v[j] = StCvt<CO>::go(v);
-            }
-            pv[q] = o;
-          }
-          vec_done = true;
-        }
-      }
-      if (!vec_done) {
-        CO *po;
-        const CS *px = nullptr;  // xpay operand of this RHS (same layout as the input)
-        size_t es;
-        if constexpr (SL == SL_AOS_PTRS) {
-          po = static_cast<CO *>(a.out_ptrs.p[slot]) + size_t(g.idx) * 4 * N;
-          if (a.xptr) px = static_cast<const CS *>(a.x_ptrs.p[slot]) + size_t(g.idx) * 4 * N;
-          es = 1;
-        } else {
-          po = static_cast<CO *>(a.out_ptrs.p[slot]) + g.idx;
-          if (a.xptr) px = static_cast<const CS *>(a.x_ptrs.p[slot]) + g.idx;
-          es = size_t(a.S);
-        }
-#pragma unroll
-        for (int sc = 0; sc < 4 * N; ++sc) {
-          cpx<A> v = acc[0][sc / N][sc % N];
-          v.re *= fin[0];
-          v.im *= fin[0];
-          if (px) {
-            const cpx<A> xx = ld_cvt<A>(px[sc * es]);
-            v.re = fmaA(fx[0], xx.re, v.re);
-            v.im = fmaA(fx[0], xx.im, v.im);
-          }
-          nrm[0] += v.re * v.re + v.im * v.im;
-          po[sc * es] = StCvt<CO>::go(v);
-        }
-      }
+      epilogue_ptrs<N, SL, CS, CO, A>(a, acc, g.idx, slot, nrm);
     }
   }
 

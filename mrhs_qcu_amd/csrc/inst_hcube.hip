@@ -31,3 +31,9 @@ QM_HC(3, 1, cb16, cb16, float, 0)
 QM_HC(3, 2, cb16, cb16, float, 0)
 QM_HC(3, 1, cb16, c32, float, 16)
 QM_HC(3, 2, cb16, c32, float, 16)
+// User AoS layout through per-RHS pointers (the qcu.h startDslash path), one
+// RHS per lane.
+#define QM_HC_AOS(N, C, A) static ::qmrhs::RegHc<N, 1, C, C, C, A, 0, ::qmrhs::SL_AOS_PTRS> QM_CAT(qm_reg_, __COUNTER__);
+QM_HC_AOS(3, c32, float)
+QM_HC_AOS(3, c64, double)
+QM_HC_AOS(3, c16, float)

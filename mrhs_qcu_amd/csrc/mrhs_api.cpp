@@ -195,6 +195,16 @@ Tile tile_for(const Geom &g, int prec, int k)
   return t;
 }
 
+Tile user_layout_tile(const Geom &g, int prec, int k)
+{
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(tune_key(g, prec | QCU_MRHS_TILE_USER_LAYOUT, k));
+    if (it != tune_cache().end()) return it->second;
+  }
+  return Tile{};
+}
+
 bool tuned(const Geom &g, int prec, int k)
 {
   std::lock_guard<std::mutex> lk(tune_mutex());
@@ -581,6 +591,7 @@ int qcuMrhsDslashUser(void *const *out_user, const void *const *in_user, const v
   c.in_ptrs = in_user;
   c.gauge = gauge;
   c.stream = static_cast<hipStream_t>(stream);
+  c.tile = user_layout_tile(g, prec, nrhs);
   if (launch_dslash(c, &err)) return api_fail(err);
   return 0;
 }
@@ -593,7 +604,8 @@ int qcuMrhsGetTile(const qcuMrhsGeom *gc, int prec, int nrhs, qcuMrhsTile *tile)
   std::string err;
   if (!geom_from_c(gc, &g, &err)) return api_fail(err);
   if (!tile) return api_fail("null tile");
-  *tile = tile_to_c(tile_for(g, prec, nrhs));
+  *tile = tile_to_c((prec & QCU_MRHS_TILE_USER_LAYOUT) ? user_layout_tile(g, prec & ~QCU_MRHS_TILE_USER_LAYOUT, nrhs)
+                                                       : tile_for(g, prec, nrhs));
   return 0;
 }
 

@@ -249,6 +249,7 @@ class Qcu {
     c.in_ptrs = ins.data();
     c.gauge = nativeGauge(diracPrec_);
     c.stream = st;
+    c.tile = qmrhs::user_layout_tile(c.geom, diracPrec_, c.k);
     std::string err;
     if (qmrhs::launch_dslash(c, &err)) fatal(err);
     CHECK_HIP(hipStreamSynchronize(st));

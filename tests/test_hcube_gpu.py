@@ -248,3 +248,45 @@ def test_hcube_cg(lib, dev):
         for prec, o in old.items():
             lib.qcuMrhsSetTile(geom, prec, k, ctypes.byref(o))
         lib.qcuMrhsSolverRelease()
+
+
+def test_hcube_user_layout(lib, dev):
+    """The user AoS layout (qcuMrhsDslashUser, the qcu.h path) through the
+    hypercube kernel, set as the user-layout tile: k = 16 and 8 (FP64 / FP32
+    / FP16-with-FP32-arithmetic are not all instantiated: FP64 and FP32 here),
+    both parities and dagger signs, 16-byte and unaligned (element-wise)
+    buffers; k = 12 keeps the site-major pointer kernel."""
+    from helpers import dev_tensor, host_complex
+
+    L, N = (8, 4, 6, 8), 3
+    g = fields.gauge_field(L, N, seed=101)
+    for prec in (_lib.DOUBLE, _lib.SINGLE):
+        for k, t in ((16, (1, 128, 1, 2, 3, 8, 0, HC | 1)), (8, (1, 64, 1, 1, 1, 0, 0, HC)),
+                     (12, (1, 128, 1, 2, 3, 8, 0, HC | 1))):
+            psi = fields.gaussian_spinors(L, N, k, seed=102 + k)
+            geom, gnat, _ = pack_inputs(lib, dev, g, psi[:1], L, N, prec)
+            _lib.check(lib.qcuMrhsSetTile(geom, prec | _lib.TILE_USER_LAYOUT, k, ctypes.byref(_lib.tile(*t))))
+            back = _lib.Tile()
+            _lib.check(lib.qcuMrhsGetTile(geom, prec | _lib.TILE_USER_LAYOUT, k, ctypes.byref(back)))
+            assert back.as_tuple() == t
+            cdt = np.complex128 if prec == _lib.DOUBLE else np.complex64
+            for offset in (0, 1):
+                # offset 1: every buffer starts one complex number into its
+                # allocation (8- or 16-byte aligned only: element-wise accesses)
+                n = psi[0].size * 2
+                ins, outs = [], []
+                for r in range(k):
+                    buf = torch.zeros(n + 2, dtype=torch.float64 if prec == _lib.DOUBLE else torch.float32,
+                                      device=dev)
+                    buf[2 * offset:2 * offset + n] = dev_tensor(psi[r].astype(cdt), dev).reshape(-1)
+                    ins.append(buf[2 * offset:2 * offset + n])
+                    outs.append(torch.zeros_like(buf)[2 * offset:2 * offset + n])
+                ai, k1 = _lib.ptr_array([t_.data_ptr() for t_ in ins])
+                ao, k2 = _lib.ptr_array([t_.data_ptr() for t_ in outs])
+                for parity, dagger in ((0, 1), (1, 0)):
+                    _lib.check(lib.qcuMrhsDslashUser(ao, ai, gnat.data_ptr(), prec, k, parity, dagger, geom,
+                                                     stream()))
+                    torch.cuda.synchronize()
+                    got = np.stack([host_complex(o.reshape(psi.shape[1:] + (2,))) for o in outs])
+                    err = rel_err(got, oracle.dslash(g, psi, L, parity, dagger))
+                    assert err < TOL[prec], (prec, k, offset, parity, err)
