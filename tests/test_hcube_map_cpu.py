@@ -1,0 +1,127 @@
+"""CPU model of the hypercube lane map (mrhs_qcu_amd/csrc/dslash_hcube.h):
+for every lane of a launch, the output site, its own inner input, the three
+corner partners (lane distance Pl, 2 Pl, 4 Pl), the outer neighbours, link
+slots, hop signs and boundary-skip flags, restated from the kernel's index
+arithmetic and checked against the canonical 8-direction neighbour table of
+the site-major kernel (dslash_kernel.h site_geometry; reference
+include/basic_data/qcu_point.cuh:33-88): every output site covered once per
+RHS slot, every (direction, sign, neighbour, link slot, live) term present
+exactly once, partners inside the same wavefront and block with the same
+slot, every index in range, and the LDS link block of each lane inside its
+workgroup's staged sites. The GPU parity of the kernels themselves is
+tests/test_hcube_gpu.py."""
+import itertools
+
+import pytest
+
+
+def decode(hb, A):
+    subLx, bLy, zb, tb, nzb = A["subLx"], A["bLy"], A["zblock"], A["tblock"], A["nzb"]
+    if A["ystream"]:
+        q, xh = divmod(hb, subLx)
+        q2, zin = divmod(q, zb)
+        q3, tin = divmod(q2, tb)
+        q4, Y = divmod(q3, bLy)
+        tout, zout = divmod(q4, nzb)
+    else:
+        q, r = divmod(hb, subLx * bLy)
+        Y, xh = divmod(r, subLx)
+        q2, zin = divmod(q, zb)
+        q3, tin = divmod(q2, tb)
+        tout, zout = divmod(q3, nzb)
+    return xh, Y, zout * zb + zin, tout * tb + tin
+
+
+def canonical(idx, A, p):
+    subLx, Ly, Lz, Lt = A["subLx"], A["Ly"], A["Lz"], A["Lt"]
+    xh = idx % subLx
+    y = idx // subLx % Ly
+    z = idx // (subLx * Ly) % Lz
+    t = idx // (subLx * Ly * Lz)
+    eo = (y + z + t) & 1
+    xf = (xh + 1) % subLx if eo != p else xh
+    xb = (xh - 1) % subLx if eo == p else xh
+    sy, sz, st = subLx, subLx * Ly, subLx * Ly * Lz
+    nb = [idx - xh + xf, idx - xh + xb,
+          idx + (-(Ly - 1) * sy if y + 1 == Ly else sy), idx + ((Ly - 1) * sy if y == 0 else -sy),
+          idx + (-(Lz - 1) * sz if z + 1 == Lz else sz), idx + ((Lz - 1) * sz if z == 0 else -sz),
+          idx + (-(Lt - 1) * st if t + 1 == Lt else st), idx + ((Lt - 1) * st if t == 0 else -st)]
+    sk = A["skip"]
+    live = [not (sk[0] and xh == subLx - 1 and p != eo), not (sk[0] and xh == 0 and p == eo),
+            not (sk[1] and y == Ly - 1), not (sk[1] and y == 0),
+            not (sk[2] and z == Lz - 1), not (sk[2] and z == 0),
+            not (sk[3] and t == Lt - 1), not (sk[3] and t == 0)]
+    return nb, live
+
+
+def check_launch(L, P, B, tb, zb, ystream, p, skip):
+    subLx, Ly, Lz, Lt = L[0] // 2, L[1], L[2], L[3]
+    S = subLx * Ly * Lz * Lt
+    A = dict(subLx=subLx, Ly=Ly, Lz=Lz, Lt=Lt, bLy=Ly // 2, bLz=Lz // 2, zblock=zb or Lz // 2, tblock=tb,
+             nzb=(Lz // 2) // (zb or Lz // 2), ystream=ystream, skip=skip)
+    lp = P.bit_length() - 1
+    lpl, gb = min(lp, 3), 3 + lp
+    nlanes = S * P
+    lanes = []
+    for gid in range(nlanes):
+        rr = gid & ((1 << gb) - 1)
+        hi, r2 = rr >> (3 + lpl), rr & ((1 << (3 + lpl)) - 1)
+        cn = r2 >> lpl
+        slot = (hi << lpl) | (r2 & ((1 << lpl) - 1))
+        sw = ((gid >> gb) << 3) | cn
+        xh, Y, Z, T = decode(sw >> 3, A)
+        y, z, t = 2 * Y + (cn & 1), 2 * Z + ((cn >> 1) & 1), 2 * T + (cn >> 2)
+        lanes.append((sw, slot, cn, xh, y, z, t, ((t * Lz + z) * Ly + y) * subLx + xh))
+    seen = set()
+    for (sw, slot, cn, xh, y, z, t, idx) in lanes:
+        assert 0 <= idx < S and slot < P
+        assert (idx, slot) not in seen
+        seen.add((idx, slot))
+    assert len(seen) == S * P
+    for blk in range((nlanes + B - 1) // B):
+        lane0, end = blk * B, min(blk * B + B, nlanes)
+        nsite = (end - lane0) >> lp
+        assert nsite * P == end - lane0 and nsite <= (B - 1) // P + 2
+        for gid in range(lane0, end):
+            assert 0 <= lanes[gid][0] - (lane0 >> lp) < nsite
+    Pl = 1 << lpl
+    for gid, (sw, slot, cn, xh, y, z, t, idx) in enumerate(lanes):
+        e = p ^ ((y + z + t) & 1)
+        sg = [-1 if e else 1, -1 if cn & 1 else 1, -1 if cn & 2 else 1, -1 if cn & 4 else 1]
+        dy, dz = subLx, subLx * Ly
+        dt = dz * Lz
+        outer = [(idx - xh if xh + 1 == subLx else idx + 1) if e else (idx + subLx - 1 if xh == 0 else idx - 1),
+                 (idx - (Ly - 1) * dy if y + 1 == Ly else idx + dy) if cn & 1 else (idx + (Ly - 1) * dy if y == 0 else idx - dy),
+                 (idx - (Lz - 1) * dz if z + 1 == Lz else idx + dz) if cn & 2 else (idx + (Lz - 1) * dz if z == 0 else idx - dz),
+                 (idx - (Lt - 1) * dt if t + 1 == Lt else idx + dt) if cn & 4 else (idx + (Lt - 1) * dt if t == 0 else idx - dt)]
+        live = [not (skip[0] and (xh == subLx - 1 if e else xh == 0)),
+                not (skip[1] and (y == Ly - 1 if cn & 1 else y == 0)),
+                not (skip[2] and (z == Lz - 1 if cn & 2 else z == 0)),
+                not (skip[3] and (t == Lt - 1 if cn & 4 else t == 0))]
+        terms = {(0, sg[0], idx, 4 if e else 0, True)}
+        for mu in (1, 2, 3):
+            partner = gid ^ (Pl << (mu - 1))
+            psw, pslot, pcn = lanes[partner][:3]
+            assert partner >> 6 == gid >> 6 and psw >> 3 == sw >> 3 and pslot == slot
+            # the partner projects with minus its own sign; this lane applies its own
+            assert -(-1 if pcn & (1 << (mu - 1)) else 1) == sg[mu]
+            terms.add((mu, sg[mu], lanes[partner][7], (4 + mu) if cn & (1 << (mu - 1)) else mu, True))
+        for mu in range(4):
+            fwd_out = -sg[mu] > 0
+            terms.add((mu, -sg[mu], outer[mu], mu if fwd_out else 4 + mu, live[mu]))
+        nb, lv = canonical(idx, A, p)
+        want = {(D >> 1, 1 if D % 2 == 0 else -1, nb[D], (D >> 1) if D % 2 == 0 else 4 + (D >> 1), lv[D])
+                for D in range(8)}
+        assert terms == want, (L, P, B, tb, zb, ystream, p, gid)
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (2, 2, 2, 2), (8, 4, 6, 2), (6, 2, 4, 8)])
+def test_hypercube_lane_map(L):
+    n = 0
+    for P, B in ((1, 64), (2, 128), (4, 64), (8, 128), (8, 256), (16, 128), (32, 256)):
+        bt, bz = L[3] // 2, L[2] // 2
+        for tb, zb in itertools.product([d for d in (1, 2, 3, 4) if bt % d == 0], [0] + [d for d in (1, 3) if bz % d == 0]):
+            for ystream, p, skip in ((0, 0, (0, 0, 0, 0)), (1, 1, (1, 1, 1, 1)), (1, 0, (1, 0, 1, 0))):
+                check_launch(L, P, B, tb, zb, ystream, p, skip)
+                n += 1
+    assert n > 0
