@@ -37,3 +37,10 @@ QM_HC(3, 2, cb16, c32, float, 16)
 QM_HC_AOS(3, c32, float)
 QM_HC_AOS(3, c64, double)
 QM_HC_AOS(3, c16, float)
+// FP8 (e4m3) spinors with binary16 links, and the FP8-sloppy solver's
+// FP8-in FP32-out operator.
+#define QM_HC_FP8(R, CO) static ::qmrhs::RegHc<3, R, ::qmrhs::c8, ::qmrhs::c16, CO, float, 16> QM_CAT(qm_reg_, __COUNTER__);
+QM_HC_FP8(2, c8)
+QM_HC_FP8(4, c8)
+QM_HC_FP8(2, c32)
+QM_HC_FP8(4, c32)

@@ -290,3 +290,62 @@ def test_hcube_user_layout(lib, dev):
                     got = np.stack([host_complex(o.reshape(psi.shape[1:] + (2,))) for o in outs])
                     err = rel_err(got, oracle.dslash(g, psi, L, parity, dagger))
                     assert err < TOL[prec], (prec, k, offset, parity, err)
+
+
+def test_hcube_tile_on_compressed_links(lib, dev):
+    """Two-row and eight-real SU(3) links with a hypercube tile (the pinned
+    24^3x48 FP32 tile is one, and the compressed-link entry points use the
+    native layout's tile): they run the site-major kernel with that blocking
+    and match the oracle on the full links."""
+    from test_gauge12_gpu import run12
+    from test_gauge8_gpu import run8
+
+    L, k = (8, 4, 6, 8), 16
+    g = fields.gauge_field(L, 3, seed=111)
+    psi = fields.gaussian_spinors(L, 3, k, seed=112)
+    t = _lib.tile(2, 128, 1, 1, 3, 8, 0, HC | 1)
+    for parity, dagger in ((0, 0), (1, 1)):
+        want = oracle.dslash(g, psi, L, parity, dagger)
+        for run in (run12, run8):
+            got = run(lib, dev, g, psi, L, parity, dagger, _lib.SINGLE, -1, tile=t)
+            assert rel_err(got, want) < TOL[1], (run.__name__, parity, dagger)
+
+
+def test_hcube_fp8(lib, dev):
+    """FP8 (e4m3) spinors with binary16 links through the hypercube kernels
+    (two and four RHS per lane): error against the FP64 oracle within the
+    stated FP8 bound (0.1), and within a few FP8 roundings of the site-major
+    FP8 kernel on the same input."""
+    from helpers import dev_tensor, host_complex
+
+    L, N, k = (8, 8, 8, 16), 3, 16
+    geom = _lib.geom(L, N)
+    g = fields.gauge_field(L, N, seed=121, smear_eps=0.3)
+    psi = fields.gaussian_spinors(L, N, k, seed=122)
+    gn = torch.empty(lib.qcuMrhsGaugeBytes(geom, _lib.FP8) // 2, dtype=torch.float16, device=dev)
+    _lib.check(lib.qcuMrhsPackGauge(gn.data_ptr(), _lib.FP8, dev_tensor(g, dev).data_ptr(), _lib.DOUBLE, geom,
+                                    stream()))
+    ins = [dev_tensor(psi[r], dev) for r in range(k)]
+    arr, keep = _lib.ptr_array([t.data_ptr() for t in ins])
+    pin = torch.empty(lib.qcuMrhsSpinorBytes(geom, k, _lib.FP8), dtype=torch.uint8, device=dev)
+    _lib.check(lib.qcuMrhsPackSpinors(pin.data_ptr(), _lib.FP8, arr, _lib.DOUBLE, k, geom, stream()))
+
+    def run(t, parity, dagger):
+        pout = torch.empty_like(pin)
+        _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gn.data_ptr(), _lib.FP8, k, parity, dagger,
+                                     geom, _lib.tile(*t), stream()))
+        outs = [torch.empty(psi.shape[1:] + (2,), dtype=torch.float64, device=dev) for _ in range(k)]
+        oarr, okeep = _lib.ptr_array([o.data_ptr() for o in outs])
+        _lib.check(lib.qcuMrhsUnpackSpinors(oarr, _lib.DOUBLE, pout.data_ptr(), _lib.FP8, k, geom, stream()))
+        torch.cuda.synchronize()
+        return np.stack([host_complex(o) for o in outs])
+
+    for parity, dagger in ((0, 0), (1, 1)):
+        want = oracle.dslash(g, psi, L, parity, dagger)
+        ref = run((2, 128, 1, 2, 0, 0, 0, 0), parity, dagger)
+        for t in ((2, 128, 1, 1, 2, 8, 0, HC | 1), (4, 128, 1, 2, 1, 0, 0, HC | 1)):
+            got = run(t, parity, dagger)
+            err = rel_err(got, want)
+            print(f"FP8 hypercube tile {t}, parity {parity}: error vs FP64 oracle {err:.3e}, "
+                  f"vs site-major FP8 {rel_err(got, ref):.3e}")
+            assert err < 0.1 and rel_err(got, ref) < 0.03, (t, parity)
