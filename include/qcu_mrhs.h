@@ -62,7 +62,9 @@ typedef struct qcuMrhsTile_s {
                        sites (neighbours inside the block shared across lanes); needs
                        P = nrhs / rhs_per_lane in {1, 2, 4, ..., 32}, block a multiple
                        of 8 P, link_lds = 1, nColors <= 4,
-                       and t_block / z_block then count blocks (divisors of Lt/2, Lz/2) */
+                       and t_block / z_block then count blocks (divisors of Lt/2, Lz/2);
+                       bits 2-3 (hypercube only): each lane walks a column of 2^bits
+                       blocks along t (t_block then counts such columns) */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);

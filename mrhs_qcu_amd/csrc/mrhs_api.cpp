@@ -130,6 +130,7 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.occ = v[14];
     t.ystream = v[15] & 1;
     t.hcube = (v[15] >> 1) & 1;
+    t.tcol = (v[15] >> 2) & 3;
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -143,7 +144,8 @@ void append_tune_file(const TuneKey &k, const Tile &t)
   if (!f) return;
   fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), std::get<0>(k),
           std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k),
-          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream | (t.hcube << 1));
+          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
+          t.ystream | (t.hcube << 1) | (t.tcol << 2));
   fclose(f);
 }
 
@@ -320,6 +322,7 @@ Tile tile_from_c(const qcuMrhsTile &c)
   t.occ = c.occupancy;
   t.ystream = c.traversal & 1;
   t.hcube = (c.traversal >> 1) & 1;
+  t.tcol = (c.traversal >> 2) & 3;
   return t;
 }
 
@@ -333,7 +336,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
   c.occupancy = t.occ;
-  c.traversal = t.ystream | (t.hcube << 1);
+  c.traversal = t.ystream | (t.hcube << 1) | (t.tcol << 2);
   return c;
 }
 
@@ -656,7 +659,8 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   auto log_tile = [&](const char *round, const Tile &t, double us) {
     if (log)
       fprintf(stderr, "qcuMrhsTune %s N=%d k=%d prec=%d tile=%d,%d,%d,%d,%d,%d,%d,%d us=%.2f\n", round, g.N, nrhs,
-              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, t.ystream | (t.hcube << 1), us);
+              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
+              t.ystream | (t.hcube << 1) | (t.tcol << 2), us);
   };
   for (const Tile &t : cands) {
     const double us = time_tile(t, 5);
