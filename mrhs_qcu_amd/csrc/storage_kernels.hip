@@ -127,50 +127,132 @@ __device__ __forceinline__ CD cvt_elem(const CSrc &v)
 }
 
 struct PtrTable {
-  void *p[64];
+  void *p[32];
 };
 
-// Pack: dst[((s*4N)+e)*k + r] = cvt(src_r[s*4N + e]); one lane per element.
+// Pack by rows: one lane per (site, spin-colour) row gathers the row's kk
+// RHS values (for each r, consecutive lanes read consecutive elements of
+// buffer r: coalesced) and writes the kk-complex row of the packed field
+// contiguously (16-byte stores when the row is 16-byte aligned). kk <= 32,
+// known at compile time up to the unrolled bound so that the pointer table
+// is indexed by constants.
 template <class CD, class CSrc>
-__global__ void __launch_bounds__(256) pack_kernel(CD *__restrict__ dst, PtrTable src, int r0, int kk, int k,
-                                                   int M /*4N*/, long long nelem /* S*M*kk */)
+__global__ void __launch_bounds__(256) pack_rows_kernel(CD *__restrict__ dst, PtrTable src, int r0, int kk, int k,
+                                                        long long nrows /* S*M */)
 {
-  const long long i = blockIdx.x * 256LL + threadIdx.x;
-  if (i >= nelem) return;
-  const int r = int(i % kk);
-  const long long se = i / kk;  // s*M + e
-  const CSrc *s = static_cast<const CSrc *>(src.p[r]);
-  dst[se * k + r0 + r] = cvt_elem<CD, CSrc>(s[se]);
-  (void)M;
+  const long long se = blockIdx.x * 256LL + threadIdx.x;
+  if (se >= nrows) return;
+  CD row[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r)
+    if (r < kk) row[r] = cvt_elem<CD, CSrc>(static_cast<const CSrc *>(src.p[r])[se]);
+  CD *d = dst + se * k + r0;
+  constexpr int V = 16 / int(sizeof(CD)) > 0 ? 16 / int(sizeof(CD)) : 1;  // complex per 16-byte store
+  if (V > 1 && kk % V == 0 && (reinterpret_cast<uintptr_t>(d) % 16) == 0) {
+#pragma unroll
+    for (int q = 0; q < 32 / V; ++q)
+      if (q * V < kk) {
+        CVec<CD, V> v;
+#pragma unroll
+        for (int j = 0; j < V; ++j) v.v[j] = row[q * V + j];
+        reinterpret_cast<CVec<CD, V> *>(d)[q] = v;
+      }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 32; ++r)
+      if (r < kk) d[r] = row[r];
+  }
 }
 
+// Unpack by rows: one lane per row reads the packed kk-complex row
+// contiguously and writes element r to buffer r (for each r, consecutive
+// lanes write consecutive elements).
 template <class CD, class CSrc>
-__global__ void __launch_bounds__(256) unpack_kernel(PtrTable dst, const CSrc *__restrict__ src, int r0, int kk, int k,
-                                                     long long nelem)
+__global__ void __launch_bounds__(256) unpack_rows_kernel(PtrTable dst, const CSrc *__restrict__ src, int r0, int kk,
+                                                          int k, long long nrows)
 {
-  const long long i = blockIdx.x * 256LL + threadIdx.x;
-  if (i >= nelem) return;
-  const int r = int(i % kk);
-  const long long se = i / kk;
-  CD *d = static_cast<CD *>(dst.p[r]);
-  d[se] = cvt_elem<CD, CSrc>(src[se * k + r0 + r]);
+  const long long se = blockIdx.x * 256LL + threadIdx.x;
+  if (se >= nrows) return;
+  const CSrc *sp = src + se * k + r0;
+  CSrc row[32];
+  constexpr int V = 16 / int(sizeof(CSrc)) > 0 ? 16 / int(sizeof(CSrc)) : 1;
+  if (V > 1 && kk % V == 0 && (reinterpret_cast<uintptr_t>(sp) % 16) == 0) {
+#pragma unroll
+    for (int q = 0; q < 32 / V; ++q)
+      if (q * V < kk) {
+        const CVec<CSrc, V> v = reinterpret_cast<const CVec<CSrc, V> *>(sp)[q];
+#pragma unroll
+        for (int j = 0; j < V; ++j) row[q * V + j] = v.v[j];
+      }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 32; ++r)
+      if (r < kk) row[r] = sp[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 32; ++r)
+    if (r < kk) static_cast<CD *>(dst.p[r])[se] = cvt_elem<CD, CSrc>(row[r]);
+}
+
+// Pack through LDS, for 2^lk RHS per launch that are the whole row (kk = k):
+// a workgroup takes 64 consecutive rows; each lane reads consecutive
+// elements of one buffer (coalesced), the tile is transposed in LDS, and the
+// 64 x kk complex destination block, contiguous, is written with
+// consecutive lanes on consecutive elements.
+template <class CD, class CSrc>
+__global__ void __launch_bounds__(256) pack_tile_kernel(CD *__restrict__ dst, PtrTable src, int lk,
+                                                        long long nrows)
+{
+  constexpr int TR = 64;  // rows per workgroup
+  __shared__ CD tile[TR * 33];
+  const int kk = 1 << lk;
+  const long long row0 = blockIdx.x * (long long)TR;
+  const int nr = (nrows - row0) < TR ? int(nrows - row0) : TR;
+  // read: element e = r * 64 + row
+  for (int e = threadIdx.x; e < TR * kk; e += 256) {
+    const int r = e >> 6, row = e & 63;
+    if (row < nr) tile[row * 33 + r] = cvt_elem<CD, CSrc>(static_cast<const CSrc *>(src.p[r])[row0 + row]);
+  }
+  __syncthreads();
+  // write: element e = row * kk + r of the contiguous block
+  CD *d = dst + row0 * kk;
+  for (int e = threadIdx.x; e < nr * kk; e += 256) d[e] = tile[(e >> lk) * 33 + (e & (kk - 1))];
 }
 
 template <class CD, class CSrc>
 static hipError_t launch_pack(bool unpack, void *packed, void *const *ptrs, int k, int M, long long S, hipStream_t st)
 {
-  for (int r0 = 0; r0 < k; r0 += 64) {
-    const int kk = (k - r0) < 64 ? (k - r0) : 64;
+  if (!unpack && k <= 32 && !(k & (k - 1))) {
+    PtrTable t{};
+    for (int r = 0; r < k; ++r) t.p[r] = ptrs[r];
+    const long long nrows = S * M;
+    const unsigned nblk = unsigned((nrows + 63) / 64);
+    hipLaunchKernelGGL((pack_tile_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(packed), t,
+                       __builtin_ctz(unsigned(k)), nrows);
+    return hipGetLastError();
+  }
+  if (!unpack) {
+    for (int r0 = 0; r0 < k; r0 += 32) {
+      const int kk = (k - r0) < 32 ? (k - r0) : 32;
+      PtrTable t{};
+      for (int r = 0; r < kk; ++r) t.p[r] = ptrs[r0 + r];
+      const long long nrows = S * M;
+      const unsigned nblk = unsigned((nrows + 255) / 256);
+      hipLaunchKernelGGL((pack_rows_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(packed), t,
+                         r0, kk, k, nrows);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  for (int r0 = 0; r0 < k; r0 += 32) {
+    const int kk = (k - r0) < 32 ? (k - r0) : 32;
     PtrTable t{};
     for (int r = 0; r < kk; ++r) t.p[r] = ptrs[r0 + r];
-    const long long nelem = S * M * kk;
-    const unsigned nblk = unsigned((nelem + 255) / 256);
-    if (!unpack)
-      hipLaunchKernelGGL((pack_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, static_cast<CD *>(packed), t, r0, kk,
-                         k, M, nelem);
-    else
-      hipLaunchKernelGGL((unpack_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, t,
-                         static_cast<const CSrc *>(packed), r0, kk, k, nelem);
+    const long long nrows = S * M;
+    const unsigned nblk = unsigned((nrows + 255) / 256);
+    hipLaunchKernelGGL((unpack_rows_kernel<CD, CSrc>), dim3(nblk), dim3(256), 0, st, t,
+                       static_cast<const CSrc *>(packed), r0, kk, k, nrows);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -229,6 +229,29 @@ def test_pack_roundtrip_and_gauge_layout(lib, dev):
     assert np.array_equal(bits[..., 1], bf16_bits(want.imag))
 
 
+@pytest.mark.parametrize("k", [1, 2, 16, 32, 5, 33, 48])
+def test_pack_unpack_every_path(lib, dev, k):
+    """Packing through the LDS tile (k a power of two <= 32; 2x2x2x6 has 24
+    sites x 12 rows = 288 rows, so the last 64-row tile is partial), by rows
+    (other k) and in chunks of 32 RHS (k > 32): packed layout and round trip
+    bit-exact in FP64, FP32 packing from FP64 equal to numpy's cast."""
+    L, N = (2, 2, 2, 6), 3
+    psi = fields.gaussian_spinors(L, N, k, seed=100 + k)
+    geom = _lib.geom(L, N)
+    ins = [dev_tensor(psi[r], dev) for r in range(k)]
+    arr, keep = _lib.ptr_array([t.data_ptr() for t in ins])
+    want = fields.to_packed(psi)
+    for prec, npt in ((_lib.DOUBLE, np.float64), (_lib.SINGLE, np.float32)):
+        pin = empty_prec(psi.shape[1:] + (k,), prec, dev)
+        _lib.check(lib.qcuMrhsPackSpinors(pin.data_ptr(), prec, arr, _lib.DOUBLE, k, geom, stream()))
+        torch.cuda.synchronize()
+        host = pin.detach().cpu().numpy()
+        assert np.array_equal(host[..., 0], want.real.astype(npt)), prec
+        assert np.array_equal(host[..., 1], want.imag.astype(npt)), prec
+        back = unpack(lib, dev, pin, prec, k, geom, psi.shape[1:])
+        assert np.array_equal(back, psi.astype(np.complex64) if prec == _lib.SINGLE else psi), prec
+
+
 def bf16_round(x):
     """Correctly rounded bfloat16 value of a double (8 significant bits,
     nearest, ties to even; the normal range of FP32), by exact scaling."""
