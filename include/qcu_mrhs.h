@@ -63,8 +63,7 @@ typedef struct qcuMrhsTile_s {
                        P = nrhs / rhs_per_lane in {1, 2, 4, ..., 32}, block a multiple
                        of 8 P, link_lds = 1, nColors <= 4,
                        and t_block / z_block then count blocks (divisors of Lt/2, Lz/2);
-                       bits 2-3 (hypercube only): each lane walks a column of 2^bits
-                       blocks along t (t_block then counts such columns) */
+                       higher bits are not defined (a tile that sets them is rejected) */
 } qcuMrhsTile;
 
 const char *qcuMrhsLastError(void);

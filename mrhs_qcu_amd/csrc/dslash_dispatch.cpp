@@ -173,8 +173,7 @@ long dslash_num_blocks(const DslashCall &c)
 {
   const long S = long(c.geom.L[0]) * c.geom.L[1] * c.geom.L[2] * c.geom.L[3] / 2;
   const int R = c.sl == SL_PACKED ? c.tile.R : 1;
-  long lanes = S * (c.k / (R > 0 ? R : 1));
-  if (runs_hcube(c)) lanes >>= c.tile.tcol;  // one lane per column of 2^tcol blocks
+  const long lanes = S * (c.k / (R > 0 ? R : 1));
   const int B = c.tile.block > 0 ? c.tile.block : 256;
   return (lanes + B - 1) / B;
 }
@@ -185,7 +184,7 @@ unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k)
   const int B = t.block > 0 ? t.block : 256;
   const long long tb = t.tblock < 1 ? 1 : t.tblock;
   const long long P = k / t.R;
-  long long lanes = 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3] * P;
+  const long long lanes = 1LL * g.L[0] / 2 * g.L[1] * g.L[2] * g.L[3] * P;
   long long half;  // lanes per half block
   if (t.hcube) {
     // Block lattice: 8 P lanes per 2x2x2x2 block, Ly / 2 blocks along y
@@ -193,7 +192,6 @@ unsigned xcd_cycle_gran(const Geom &g, const Tile &t, int k)
     const long long zb = t.zblock < 1 ? g.L[2] / 2 : t.zblock;
     const long long by = g.L[1] / 2;
     half = tb * zb * (by % 2 ? by : by / 2) * (g.L[0] / 2) * 8 * P;
-    lanes >>= t.tcol;  // one lane per column of 2^tcol blocks
   } else {
     const long long zb = t.zblock < 1 ? g.L[2] : t.zblock;
     half = tb * zb * (g.L[1] / 2) * (g.L[0] / 2) * P;
@@ -253,7 +251,6 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
     a.hdiv_row = make_fastdiv(uint32_t(a.subLx * a.bLy));
     a.hdiv_nzb = make_fastdiv(uint32_t(bz / a.zblock));
     a.hc_lp = uint32_t(__builtin_ctz(unsigned(P)));
-    a.hc_m = 1u << c.tile.tcol;
   }
   KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
   key.hc = hc;
@@ -267,7 +264,7 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   a.k = c.k;
   a.P = c.k / R;
   a.div_P = make_fastdiv(uint32_t(a.P));
-  const unsigned long long lanes = (1ULL * a.S * a.P) >> (hc ? c.tile.tcol : 0);
+  const unsigned long long lanes = 1ULL * a.S * a.P;
   if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
   a.nlanes = uint32_t(lanes);
   a.nblocks = uint32_t((lanes + B - 1) / B);
@@ -315,6 +312,7 @@ static int check_call(const DslashCall &c, std::string *err)
                      "sides and tiles with rhs_per_lane 2 or 4 and LDS-staged links"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
+  if (c.tile.trav_hi) return fail(err, "traversal bits 2 and above are not defined"), 1;
   if (!c.tile.hcube) {
     if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
     if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
@@ -322,10 +320,8 @@ static int check_call(const DslashCall &c, std::string *err)
     // Counted in 2x2x2x2 blocks (so they also divide Lt and Lz when the
     // site-major kernel runs the tile).
     if (c.sl == SL_SOA_PTRS) return fail(err, "hypercube tiles are for the packed and user AoS layouts"), 1;
-    if (c.tile.tcol < 0 || c.tile.tcol > 3 || (c.geom.L[3] / 2) % (1 << c.tile.tcol))
-      return fail(err, "hypercube t columns of 2^tcol blocks must divide Lt / 2"), 1;
-    if (c.tile.tblock > 1 && ((c.geom.L[3] / 2) >> c.tile.tcol) % c.tile.tblock)
-      return fail(err, "hypercube t_block must divide Lt / 2 (in t columns of blocks)"), 1;
+    if (c.tile.tblock > 1 && (c.geom.L[3] / 2) % c.tile.tblock)
+      return fail(err, "hypercube t_block must divide Lt / 2"), 1;
     if (c.tile.zblock > 0 && (c.geom.L[2] / 2) % c.tile.zblock)
       return fail(err, "hypercube z_block must divide Lz / 2"), 1;
   }
@@ -469,12 +465,10 @@ int launch_dslash(const DslashCall &c, std::string *err)
         b.hdiv_row = make_fastdiv(uint32_t(b.subLx * b.bLy));
         b.hdiv_nzb = make_fastdiv(uint32_t(bz / b.zblock));
         b.hc_lp = uint32_t(__builtin_ctz(unsigned(kk)));
-        b.hc_m = 1;
         b.nblocks = uint32_t((b.nlanes + uint32_t(Bh) - 1) / uint32_t(Bh));
         b.xcd_map = (c.tile.hints & 4) ? 0 : 1;
         Tile t = c.tile;
         t.R = 1;
-        t.tcol = 0;
         b.xcd_gran = xcd_cycle_gran(c.geom, t, kk);
         f = fh;
         shm = lds_h;

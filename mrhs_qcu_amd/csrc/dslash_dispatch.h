@@ -108,8 +108,7 @@ struct Tile {
   int ystream = 0; // 1: stream each (tblock, zblock) block along y
   int hcube = 0;   // 1: hypercube work mapping (2x2x2x2 blocks, one per 8 P lanes; tblock / zblock
                    // then count blocks)
-  int tcol = 0;    // hypercube: each lane walks a column of 2^tcol blocks along t (tblock then counts
-                   // such columns)
+  int trav_hi = 0; // traversal bits 2 and above: none defined (a tile that sets them is rejected)
 };
 
 struct DslashCall {

@@ -74,14 +74,11 @@ struct HcSite {
   int idx, xh, y, z, t;
 };
 
-// Output site number sw = 8 * column + corner (corner bits: y, z, t) in
-// block j of its column: the traversal walks columns of hc_m blocks along t,
-// so the block's t index is hc_m * (the column's t index) + j.
-__device__ __forceinline__ HcSite hc_site(uint32_t sw, const DslashArgs &a, uint32_t j)
+// Output site number sw = 8 * block + corner (corner bits: y, z, t).
+__device__ __forceinline__ HcSite hc_site(uint32_t sw, const DslashArgs &a)
 {
   uint32_t xh, Y, Z, T;
   hc_decode(sw >> 3, a, xh, Y, Z, T);
-  T = T * a.hc_m + j;
   const uint32_t c = sw & 7u;
   HcSite s;
   s.xh = int(xh);
@@ -283,7 +280,7 @@ __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;
-  // Lanes of one column of blocks: 8 P = 2^gb. With P <= 8 (Pl = P) the block is 8 P
+  // Lanes of one block: 8 P = 2^gb. With P <= 8 (Pl = P) the block is 8 P
   // consecutive lanes, corner-major; with P > 8 (Pl = 8) its RHS slots are
   // split over P / 8 wavefronts, each holding all 8 corners for 8 slots
   // (lane = 64 hi + 8 corner + lo, slot = 8 hi + lo), so the corner partners
@@ -296,114 +293,111 @@ __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_
   const uint32_t r2 = rr & ((1u << (3u + lpl)) - 1u);
   const uint32_t cn = r2 >> lpl;
   const uint32_t slot = (hi << lpl) | (r2 & ((1u << lpl) - 1u));
-  const uint32_t sw = ((gid >> gb) << 3) | cn;  // output site number: 8 * column + corner
+  const uint32_t sw = ((gid >> gb) << 3) | cn;  // output site number: 8 * block + corner
   const uint32_t Pl = 1u << lpl;
   constexpr int LC = N * N;
+  {
+    // The 8 links of every output site of this workgroup, once, into LDS
+    // (workgroups hold whole blocks: the block size divides the workgroup).
+    const uint32_t first = lane0 >> lp;
+    const uint32_t end = (lane0 + blockDim.x < a.nlanes) ? lane0 + blockDim.x : a.nlanes;
+    const uint32_t nsite = (end - lane0) >> lp;
+    constexpr int chunk16 = (8 * LC * int(sizeof(CG))) / 16;
+    static_assert((8 * LC * int(sizeof(CG))) % 16 == 0, "site link block in 16-byte pieces");
+    u32x4 *dst = reinterpret_cast<u32x4 *>(smem);
+    const u32x4 *src = static_cast<const u32x4 *>(a.gauge);
+    for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
+      const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
+      const uint32_t phys = uint32_t(hc_site(first + sloc, a).idx);
+      const u32x4 *ps = src + (size_t(a.parity) * a.S + phys) * chunk16 + w;
+      dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
+    }
+    if (a.efac_off) {
+      A *ef = reinterpret_cast<A *>(smem + a.efac_off);
+      for (int i = threadIdx.x; i < a.k; i += blockDim.x) {
+        ef[i] = A(a.xb) * (a.sc_in ? A(a.sc_in[i]) : A(1));
+        ef[a.k + i] = A(a.xa) * (a.sc_x ? A(a.sc_x[i]) : A(1));
+        ef[2 * a.k + i] = a.sc_out ? A(1) / A(a.sc_out[i]) : A(1);
+        ef[3 * a.k + i] = a.rx_alpha ? A(a.rx_alpha[i]) : A(0);
+      }
+    }
+    __syncthreads();
+  }
   A nrm[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) nrm[r] = A(0);
-  for (uint32_t j = 0; j < a.hc_m; ++j) {
-    if (j) __syncthreads();  // every lane is done with the previous block's links
-    {
-      // The 8 links of every output site of this workgroup in block j into
-      // LDS (workgroups hold whole columns: 8 P lanes divide the workgroup).
-      const uint32_t first = lane0 >> lp;
-      const uint32_t end = (lane0 + blockDim.x < a.nlanes) ? lane0 + blockDim.x : a.nlanes;
-      const uint32_t nsite = (end - lane0) >> lp;
-      constexpr int chunk16 = (8 * LC * int(sizeof(CG))) / 16;
-      static_assert((8 * LC * int(sizeof(CG))) % 16 == 0, "site link block in 16-byte pieces");
-      u32x4 *dst = reinterpret_cast<u32x4 *>(smem);
-      const u32x4 *src = static_cast<const u32x4 *>(a.gauge);
-      for (uint32_t i = threadIdx.x; i < nsite * chunk16; i += blockDim.x) {
-        const uint32_t sloc = i / chunk16, w = i - sloc * chunk16;
-        const uint32_t phys = uint32_t(hc_site(first + sloc, a, j).idx);
-        const u32x4 *ps = src + (size_t(a.parity) * a.S + phys) * chunk16 + w;
-        dst[i] = a.nt_link ? __builtin_nontemporal_load(ps) : *ps;
-      }
-      if (j == 0 && a.efac_off) {
-        A *ef = reinterpret_cast<A *>(smem + a.efac_off);
-        for (int i = threadIdx.x; i < a.k; i += blockDim.x) {
-          ef[i] = A(a.xb) * (a.sc_in ? A(a.sc_in[i]) : A(1));
-          ef[a.k + i] = A(a.xa) * (a.sc_x ? A(a.sc_x[i]) : A(1));
-          ef[2 * a.k + i] = a.sc_out ? A(1) / A(a.sc_out[i]) : A(1);
-          ef[3 * a.k + i] = a.rx_alpha ? A(a.rx_alpha[i]) : A(0);
-        }
-      }
-      __syncthreads();
+  if (active) {
+    const HcSite st = hc_site(sw, a);
+    const int idx = st.idx;
+    // Full x = 2 xh + e: the inner x neighbour is ahead (X+) for e = 0.
+    const int e = a.parity ^ ((st.y + st.z + st.t) & 1);
+    const A s = A(a.sign);
+    const A sx = e ? -s : s, sy = (cn & 1u) ? -s : s, sz = (cn & 2u) ? -s : s, stt = (cn & 4u) ? -s : s;
+    const int dy = a.subLx, dz = a.subLx * a.Ly, dt = dz * a.Lz;
+    // Outer neighbours (periodic wrap) and whether their term is live.
+    const int nx = e ? (st.xh + 1 == a.subLx ? idx - st.xh : idx + 1) : (st.xh == 0 ? idx + a.subLx - 1 : idx - 1);
+    const int ny = (cn & 1u) ? (st.y + 1 == a.Ly ? idx - (a.Ly - 1) * dy : idx + dy)
+                             : (st.y == 0 ? idx + (a.Ly - 1) * dy : idx - dy);
+    const int nz = (cn & 2u) ? (st.z + 1 == a.Lz ? idx - (a.Lz - 1) * dz : idx + dz)
+                             : (st.z == 0 ? idx + (a.Lz - 1) * dz : idx - dz);
+    const int nt = (cn & 4u) ? (st.t + 1 == a.Lt ? idx - (a.Lt - 1) * dt : idx + dt)
+                             : (st.t == 0 ? idx + (a.Lt - 1) * dt : idx - dt);
+    const bool lx = !(a.skip[0] && (e ? st.xh == a.subLx - 1 : st.xh == 0));
+    const bool ly = !(a.skip[1] && ((cn & 1u) ? st.y == a.Ly - 1 : st.y == 0));
+    const bool lz = !(a.skip[2] && ((cn & 2u) ? st.z == a.Lz - 1 : st.z == 0));
+    const bool lt = !(a.skip[3] && ((cn & 4u) ? st.t == a.Lt - 1 : st.t == 0));
+    const CG *blk8 = reinterpret_cast<const CG *>(smem) + size_t(sw - (lane0 >> lp)) * 8 * LC;
+    const A *ef = a.efac_off ? reinterpret_cast<const A *>(smem + a.efac_off) : nullptr;
+
+    cpx<A> acc[R][4][N];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+        for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};
+
+    constexpr bool kPair = SL == SL_PACKED && std::is_same<CS, CG>::value &&
+                           (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) &&
+                           std::is_same<A, float>::value;
+    // Inner hops: x from the own spinor, y / z / t through the partners;
+    // then the outer hops (the opposite direction of the inner one), one load
+    // ahead. Link slots: forward mu, backward 4 + mu.
+    if constexpr (kPair) {
+      using PV = typename Pair<CS>::V;
+      PV psA[R][4][N], psB[R][4][N];
+      load_spinor_p<CS, N, R>(psA, a, idx, int(slot));  // own inner site
+      load_spinor_p<CS, N, R>(psB, a, nx, int(slot));   // x outer
+      hc_hop_p<CS, N, 0, R>(acc, psA, blk8, e ? 4 : 0, sx);
+      hc_hop_xchg_p<CS, N, 1, R>(acc, psA, blk8, (cn & 1u) ? 5 : 1, sy, int(Pl));
+      hc_hop_xchg_p<CS, N, 2, R>(acc, psA, blk8, (cn & 2u) ? 6 : 2, sz, int(2 * Pl));
+      hc_hop_xchg_p<CS, N, 3, R>(acc, psA, blk8, (cn & 4u) ? 7 : 3, stt, int(4 * Pl));
+      load_spinor_p<CS, N, R>(psA, a, ny, int(slot));
+      if (lx) hc_hop_p<CS, N, 0, R>(acc, psB, blk8, e ? 0 : 4, -sx);
+      load_spinor_p<CS, N, R>(psB, a, nz, int(slot));
+      if (ly) hc_hop_p<CS, N, 1, R>(acc, psA, blk8, (cn & 1u) ? 1 : 5, -sy);
+      load_spinor_p<CS, N, R>(psA, a, nt, int(slot));
+      if (lz) hc_hop_p<CS, N, 2, R>(acc, psB, blk8, (cn & 2u) ? 2 : 6, -sz);
+      if (lt) hc_hop_p<CS, N, 3, R>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
+    } else {
+      cpx<A> psA[R][4][N], psB[R][4][N];
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, idx, int(slot));  // own inner site
+      load_spinor<N, R, SL, CS, A>(psB, a, a.in, nx, int(slot));   // x outer
+      hc_hop<N, 0, R, CG, A>(acc, psA, blk8, e ? 4 : 0, sx);
+      hc_hop_xchg<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 5 : 1, sy, int(Pl));
+      hc_hop_xchg<N, 2, R, CG, A>(acc, psA, blk8, (cn & 2u) ? 6 : 2, sz, int(2 * Pl));
+      hc_hop_xchg<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 7 : 3, stt, int(4 * Pl));
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, ny, int(slot));
+      if (lx) hc_hop<N, 0, R, CG, A>(acc, psB, blk8, e ? 0 : 4, -sx);
+      load_spinor<N, R, SL, CS, A>(psB, a, a.in, nz, int(slot));
+      if (ly) hc_hop<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 1 : 5, -sy);
+      load_spinor<N, R, SL, CS, A>(psA, a, a.in, nt, int(slot));
+      if (lz) hc_hop<N, 2, R, CG, A>(acc, psB, blk8, (cn & 2u) ? 2 : 6, -sz);
+      if (lt) hc_hop<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
     }
-    if (active) {
-      const HcSite st = hc_site(sw, a, j);
-      const int idx = st.idx;
-      // Full x = 2 xh + e: the inner x neighbour is ahead (X+) for e = 0.
-      const int e = a.parity ^ ((st.y + st.z + st.t) & 1);
-      const A s = A(a.sign);
-      const A sx = e ? -s : s, sy = (cn & 1u) ? -s : s, sz = (cn & 2u) ? -s : s, stt = (cn & 4u) ? -s : s;
-      const int dy = a.subLx, dz = a.subLx * a.Ly, dt = dz * a.Lz;
-      // Outer neighbours (periodic wrap) and whether their term is live.
-      const int nx = e ? (st.xh + 1 == a.subLx ? idx - st.xh : idx + 1) : (st.xh == 0 ? idx + a.subLx - 1 : idx - 1);
-      const int ny = (cn & 1u) ? (st.y + 1 == a.Ly ? idx - (a.Ly - 1) * dy : idx + dy)
-                               : (st.y == 0 ? idx + (a.Ly - 1) * dy : idx - dy);
-      const int nz = (cn & 2u) ? (st.z + 1 == a.Lz ? idx - (a.Lz - 1) * dz : idx + dz)
-                               : (st.z == 0 ? idx + (a.Lz - 1) * dz : idx - dz);
-      const int nt = (cn & 4u) ? (st.t + 1 == a.Lt ? idx - (a.Lt - 1) * dt : idx + dt)
-                               : (st.t == 0 ? idx + (a.Lt - 1) * dt : idx - dt);
-      const bool lx = !(a.skip[0] && (e ? st.xh == a.subLx - 1 : st.xh == 0));
-      const bool ly = !(a.skip[1] && ((cn & 1u) ? st.y == a.Ly - 1 : st.y == 0));
-      const bool lz = !(a.skip[2] && ((cn & 2u) ? st.z == a.Lz - 1 : st.z == 0));
-      const bool lt = !(a.skip[3] && ((cn & 4u) ? st.t == a.Lt - 1 : st.t == 0));
-      const CG *blk8 = reinterpret_cast<const CG *>(smem) + size_t(sw - (lane0 >> lp)) * 8 * LC;
-      const A *ef = a.efac_off ? reinterpret_cast<const A *>(smem + a.efac_off) : nullptr;
 
-      cpx<A> acc[R][4][N];
-  #pragma unroll
-      for (int r = 0; r < R; ++r)
-  #pragma unroll
-        for (int sp = 0; sp < 4; ++sp)
-  #pragma unroll
-          for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};
-
-      constexpr bool kPair = SL == SL_PACKED && std::is_same<CS, CG>::value &&
-                             (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) &&
-                             std::is_same<A, float>::value;
-      // Inner hops: x from the own spinor, y / z / t through the partners;
-      // then the outer hops (the opposite direction of the inner one), one load
-      // ahead. Link slots: forward mu, backward 4 + mu.
-      if constexpr (kPair) {
-        using PV = typename Pair<CS>::V;
-        PV psA[R][4][N], psB[R][4][N];
-        load_spinor_p<CS, N, R>(psA, a, idx, int(slot));  // own inner site
-        load_spinor_p<CS, N, R>(psB, a, nx, int(slot));   // x outer
-        hc_hop_p<CS, N, 0, R>(acc, psA, blk8, e ? 4 : 0, sx);
-        hc_hop_xchg_p<CS, N, 1, R>(acc, psA, blk8, (cn & 1u) ? 5 : 1, sy, int(Pl));
-        hc_hop_xchg_p<CS, N, 2, R>(acc, psA, blk8, (cn & 2u) ? 6 : 2, sz, int(2 * Pl));
-        hc_hop_xchg_p<CS, N, 3, R>(acc, psA, blk8, (cn & 4u) ? 7 : 3, stt, int(4 * Pl));
-        load_spinor_p<CS, N, R>(psA, a, ny, int(slot));
-        if (lx) hc_hop_p<CS, N, 0, R>(acc, psB, blk8, e ? 0 : 4, -sx);
-        load_spinor_p<CS, N, R>(psB, a, nz, int(slot));
-        if (ly) hc_hop_p<CS, N, 1, R>(acc, psA, blk8, (cn & 1u) ? 1 : 5, -sy);
-        load_spinor_p<CS, N, R>(psA, a, nt, int(slot));
-        if (lz) hc_hop_p<CS, N, 2, R>(acc, psB, blk8, (cn & 2u) ? 2 : 6, -sz);
-        if (lt) hc_hop_p<CS, N, 3, R>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
-      } else {
-        cpx<A> psA[R][4][N], psB[R][4][N];
-        load_spinor<N, R, SL, CS, A>(psA, a, a.in, idx, int(slot));  // own inner site
-        load_spinor<N, R, SL, CS, A>(psB, a, a.in, nx, int(slot));   // x outer
-        hc_hop<N, 0, R, CG, A>(acc, psA, blk8, e ? 4 : 0, sx);
-        hc_hop_xchg<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 5 : 1, sy, int(Pl));
-        hc_hop_xchg<N, 2, R, CG, A>(acc, psA, blk8, (cn & 2u) ? 6 : 2, sz, int(2 * Pl));
-        hc_hop_xchg<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 7 : 3, stt, int(4 * Pl));
-        load_spinor<N, R, SL, CS, A>(psA, a, a.in, ny, int(slot));
-        if (lx) hc_hop<N, 0, R, CG, A>(acc, psB, blk8, e ? 0 : 4, -sx);
-        load_spinor<N, R, SL, CS, A>(psB, a, a.in, nz, int(slot));
-        if (ly) hc_hop<N, 1, R, CG, A>(acc, psA, blk8, (cn & 1u) ? 1 : 5, -sy);
-        load_spinor<N, R, SL, CS, A>(psA, a, a.in, nt, int(slot));
-        if (lz) hc_hop<N, 2, R, CG, A>(acc, psB, blk8, (cn & 2u) ? 2 : 6, -sz);
-        if (lt) hc_hop<N, 3, R, CG, A>(acc, psA, blk8, (cn & 4u) ? 3 : 7, -stt);
-      }
-
-      if constexpr (SL == SL_PACKED) epilogue_packed<N, R, CS, CO, A>(a, acc, idx, slot, ef, nrm);
-      else epilogue_ptrs<N, SL, CS, CO, A>(a, acc, idx, slot, nrm);
-    }
+    if constexpr (SL == SL_PACKED) epilogue_packed<N, R, CS, CO, A>(a, acc, idx, slot, ef, nrm);
+    else epilogue_ptrs<N, SL, CS, CO, A>(a, acc, idx, slot, nrm);
   }
   workgroup_partials<R, A>(a, blk, active, slot, nrm);
 }

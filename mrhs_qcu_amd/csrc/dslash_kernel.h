@@ -97,7 +97,6 @@ struct DslashArgs {
   // 2x2x2x2 blocks (subLx x Ly/2 x Lz/2 x Lt/2) in the traversal order above,
   // with tblock / zblock counted in blocks; 2^hc_lp lanes per site.
   uint32_t hc_lp;
-  uint32_t hc_m;  // blocks per lane along t (a column of hc_m blocks; the traversal walks columns)
   int bLy, bLz;
   FastDiv hdiv_Ly, hdiv_row, hdiv_nzb;
   // Pointer layouts only. Kept last: the packed kernels read only the fields

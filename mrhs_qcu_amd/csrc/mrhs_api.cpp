@@ -130,7 +130,7 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.occ = v[14];
     t.ystream = v[15] & 1;
     t.hcube = (v[15] >> 1) & 1;
-    t.tcol = (v[15] >> 2) & 3;
+    if (v[15] >> 2) continue;  // traversal bits 2 and above: not defined
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -145,7 +145,7 @@ void append_tune_file(const TuneKey &k, const Tile &t)
   fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), std::get<0>(k),
           std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k),
           std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
-          t.ystream | (t.hcube << 1) | (t.tcol << 2));
+          t.ystream | (t.hcube << 1));
   fclose(f);
 }
 
@@ -322,7 +322,7 @@ Tile tile_from_c(const qcuMrhsTile &c)
   t.occ = c.occupancy;
   t.ystream = c.traversal & 1;
   t.hcube = (c.traversal >> 1) & 1;
-  t.tcol = (c.traversal >> 2) & 3;
+  t.trav_hi = c.traversal >> 2;
   return t;
 }
 
@@ -336,7 +336,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
   c.occupancy = t.occ;
-  c.traversal = t.ystream | (t.hcube << 1) | (t.tcol << 2);
+  c.traversal = t.ystream | (t.hcube << 1);
   return c;
 }
 
@@ -622,6 +622,7 @@ int qcuMrhsSetTile(const qcuMrhsGeom *gc, int prec, int nrhs, const qcuMrhsTile 
   const Tile t = tile_from_c(*tile);
   if (t.R != 1 && t.R != 2 && t.R != 4) return api_fail("rhs_per_lane must be 1, 2 or 4");
   if (t.block < 64 || t.block > 256 || t.block % 64) return api_fail("block must be 64, 128, 192 or 256");
+  if (t.trav_hi) return api_fail("traversal bits 2 and above are not defined");
   set_tile(g, prec, nrhs, t);
   return 0;
 }
@@ -660,7 +661,7 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
     if (log)
       fprintf(stderr, "qcuMrhsTune %s N=%d k=%d prec=%d tile=%d,%d,%d,%d,%d,%d,%d,%d us=%.2f\n", round, g.N, nrhs,
               prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
-              t.ystream | (t.hcube << 1) | (t.tcol << 2), us);
+              t.ystream | (t.hcube << 1), us);
   };
   for (const Tile &t : cands) {
     const double us = time_tile(t, 5);

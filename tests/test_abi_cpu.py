@@ -318,4 +318,6 @@ def test_set_tile_round_trip(lib):
     assert b"rhs_per_lane" in lib.qcuMrhsLastError()
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 96))) != 0
     assert b"block" in lib.qcuMrhsLastError()
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 128, 1, 1, 1, 8, 0, 3 | 4))) != 0
+    assert b"traversal bits" in lib.qcuMrhsLastError()
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, None) != 0

@@ -54,26 +54,14 @@ def canonical(idx, A, p):
     return nb, live
 
 
-def check_launch(L, P, B, tb, zb, ystream, p, skip, m=1):
-    """One launch; m: blocks per lane along t (a t column of m blocks)."""
-    for j in range(m):
-        check_block(L, P, B, tb, zb, ystream, p, skip, m, j)
-    # every (site, RHS slot) exactly once over the column's blocks
-    seen = set()
-    for j in range(m):
-        for (sw, slot, cn, xh, y, z, t, idx) in lane_sites(L, P, tb, zb, ystream, m, j)[0]:
-            assert (idx, slot) not in seen
-            seen.add((idx, slot))
-    assert len(seen) == L[0] // 2 * L[1] * L[2] * L[3] * P
-
-
-def lane_sites(L, P, tb, zb, ystream, m, j, skip=(0, 0, 0, 0)):
+def check_launch(L, P, B, tb, zb, ystream, p, skip):
     subLx, Ly, Lz, Lt = L[0] // 2, L[1], L[2], L[3]
+    S = subLx * Ly * Lz * Lt
     A = dict(subLx=subLx, Ly=Ly, Lz=Lz, Lt=Lt, bLy=Ly // 2, bLz=Lz // 2, zblock=zb or Lz // 2, tblock=tb,
              nzb=(Lz // 2) // (zb or Lz // 2), ystream=ystream, skip=skip)
     lp = P.bit_length() - 1
     lpl, gb = min(lp, 3), 3 + lp
-    nlanes = subLx * Ly * Lz * Lt * P // m
+    nlanes = S * P
     lanes = []
     for gid in range(nlanes):
         rr = gid & ((1 << gb) - 1)
@@ -81,23 +69,15 @@ def lane_sites(L, P, tb, zb, ystream, m, j, skip=(0, 0, 0, 0)):
         cn = r2 >> lpl
         slot = (hi << lpl) | (r2 & ((1 << lpl) - 1))
         sw = ((gid >> gb) << 3) | cn
-        xh, Y, Z, Tc = decode(sw >> 3, A)
-        T = Tc * m + j
+        xh, Y, Z, T = decode(sw >> 3, A)
         y, z, t = 2 * Y + (cn & 1), 2 * Z + ((cn >> 1) & 1), 2 * T + (cn >> 2)
         lanes.append((sw, slot, cn, xh, y, z, t, ((t * Lz + z) * Ly + y) * subLx + xh))
-    return lanes, A, nlanes, lp, lpl
-
-
-def check_block(L, P, B, tb, zb, ystream, p, skip, m, j):
-    subLx, Ly, Lz, Lt = L[0] // 2, L[1], L[2], L[3]
-    S = subLx * Ly * Lz * Lt
-    lanes, A, nlanes, lp, lpl = lane_sites(L, P, tb, zb, ystream, m, j, skip)
     seen = set()
     for (sw, slot, cn, xh, y, z, t, idx) in lanes:
         assert 0 <= idx < S and slot < P
         assert (idx, slot) not in seen
         seen.add((idx, slot))
-    assert len(seen) == S * P // m
+    assert len(seen) == S * P
     for blk in range((nlanes + B - 1) // B):
         lane0, end = blk * B, min(blk * B + B, nlanes)
         nsite = (end - lane0) >> lp
@@ -135,17 +115,13 @@ def check_block(L, P, B, tb, zb, ystream, p, skip, m, j):
         assert terms == want, (L, P, B, tb, zb, ystream, p, gid)
 
 
-@pytest.mark.parametrize("L", [(4, 4, 4, 4), (2, 2, 2, 2), (8, 4, 6, 2), (6, 2, 4, 8), (4, 2, 2, 16)])
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (2, 2, 2, 2), (8, 4, 6, 2), (6, 2, 4, 8)])
 def test_hypercube_lane_map(L):
     n = 0
     for P, B in ((1, 64), (2, 128), (4, 64), (8, 128), (8, 256), (16, 128), (32, 256)):
-        for m in (1, 2, 4):
-            bt, bz = L[3] // 2, L[2] // 2
-            if bt % m:
-                continue
-            for tb, zb in itertools.product([d for d in (1, 2, 3, 4) if (bt // m) % d == 0],
-                                            [0] + [d for d in (1, 3) if bz % d == 0]):
-                for ystream, p, skip in ((0, 0, (0, 0, 0, 0)), (1, 1, (1, 1, 1, 1)), (1, 0, (1, 0, 1, 0))):
-                    check_launch(L, P, B, tb, zb, ystream, p, skip, m)
-                    n += 1
+        bt, bz = L[3] // 2, L[2] // 2
+        for tb, zb in itertools.product([d for d in (1, 2, 3, 4) if bt % d == 0], [0] + [d for d in (1, 3) if bz % d == 0]):
+            for ystream, p, skip in ((0, 0, (0, 0, 0, 0)), (1, 1, (1, 1, 1, 1)), (1, 0, (1, 0, 1, 0))):
+                check_launch(L, P, B, tb, zb, ystream, p, skip)
+                n += 1
     assert n > 0
