@@ -98,33 +98,30 @@ __device__ __forceinline__ void project(cpx<A> (&h)[2][N], const cpx<A> (&psi)[4
   }
 }
 
-// acc[4][N] = the 4-spinor rebuilt from the half spinor g of direction D.
-template <int N, class A>
-__device__ __forceinline__ void reconstruct(cpx<A> (&acc)[4][N], const cpx<A> (&g)[2][N], int D, A s)
+// out[4][N] (RHS stride k) += the 4-spinor rebuilt from the half spinor g of
+// direction D. D is a template argument so that every spin row index is a
+// constant (a per-lane row index would put the spinor in scratch memory).
+template <int D, int N, class C, class A>
+__device__ __forceinline__ void add_reconstructed(C *po, const cpx<A> (&g)[2][N], A s, int k)
 {
-  switch (D) {
-#define QM_RECON(DD)                                                                   \
-  case DD: {                                                                           \
-    constexpr HopEntry e = hop_entry(DD);                                              \
-    _Pragma("unroll") for (int i = 0; i < N; ++i)                                      \
-    {                                                                                  \
-      acc[0][i] = g[0][i];                                                             \
-      acc[1][i] = g[1][i];                                                             \
-      acc[e.a0][i] = add_phase<ConjPhase<e.k0>::v>(cpx<A>{A(0), A(0)}, g[0][i], s);    \
-      acc[e.a1][i] = add_phase<ConjPhase<e.k1>::v>(cpx<A>{A(0), A(0)}, g[1][i], s);    \
-    }                                                                                  \
-    break;                                                                             \
+  constexpr HopEntry e = hop_entry(D);
+  cpx<A> add[4][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    add[0][i] = g[0][i];
+    add[1][i] = g[1][i];
+    add[e.a0][i] = add_phase<ConjPhase<e.k0>::v>(cpx<A>{A(0), A(0)}, g[0][i], s);
+    add[e.a1][i] = add_phase<ConjPhase<e.k1>::v>(cpx<A>{A(0), A(0)}, g[1][i], s);
   }
-    QM_RECON(0)
-    QM_RECON(1)
-    QM_RECON(2)
-    QM_RECON(3)
-    QM_RECON(4)
-    QM_RECON(5)
-    QM_RECON(6)
-    QM_RECON(7)
-#undef QM_RECON
-  }
+#pragma unroll
+  for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      cpx<A> v = ld_cvt<A>(po[size_t(sp * N + c) * k]);
+      v.re += add[sp][c].re;
+      v.im += add[sp][c].im;
+      po[size_t(sp * N + c) * k] = StCvt<C>::go(v);
+    }
 }
 
 // y = V x (dag = false) or V^+ x (dag = true) for the two colour vectors.
@@ -219,18 +216,21 @@ __global__ void __launch_bounds__(256) dslash_faces_kernel(C *__restrict__ out, 
       for (int c = 0; c < N; ++c) o[j][c] = h[j][c];
     D = 2 * g.mu + 1;
   }
-  cpx<A> add[4][N];
-  reconstruct<N, A>(add, o, D, s);
   C *po = out + size_t(site) * 4 * N * g.k + r;
-#pragma unroll
-  for (int sp = 0; sp < 4; ++sp)
-#pragma unroll
-    for (int c = 0; c < N; ++c) {
-      cpx<A> v = ld_cvt<A>(po[size_t(sp * N + c) * g.k]);
-      v.re += add[sp][c].re;
-      v.im += add[sp][c].im;
-      po[size_t(sp * N + c) * g.k] = StCvt<C>::go(v);
-    }
+  // D is uniform over the launch (2 mu or 2 mu + 1 by workgroup row).
+  switch (D) {
+#define QM_ADD(DD) \
+  case DD: add_reconstructed<DD, N, C, A>(po, o, s, g.k); break;
+    QM_ADD(0)
+    QM_ADD(1)
+    QM_ADD(2)
+    QM_ADD(3)
+    QM_ADD(4)
+    QM_ADD(5)
+    QM_ADD(6)
+    QM_ADD(7)
+#undef QM_ADD
+  }
 }
 
 FaceGeo face_geo(const Geom &geom, int mu, int k, int parity, int dagger)

@@ -1,0 +1,76 @@
+"""Register and scratch budgets of the kernels the pinned tiles and the CG run.
+
+These kernels are bound by the memory system and keep their speed only with
+their waves per SIMD: one wave fewer per SIMD took the FP16 24^3x48 operator
+from 190 to 276 us. The figures come from the AMDGPU metadata of the built
+libqcu.so (no GPU needed). A change that raises a kernel's register count past
+its budget fails here, so the budget is raised deliberately, with the kernel
+re-timed, or the change is reworked.
+
+Budgets are the measured VGPR counts rounded up to the 8-register allocation
+granule. Waves per SIMD on gfx950: 3 up to 168 VGPRs, 2 up to 248 (the
+compiler reports one wave at 256).
+"""
+import os
+import re
+
+import pytest
+
+import kernel_meta
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mrhs_qcu_amd", "libqcu.so")
+
+HC = "_ZN5qmrhs16dslash_hc_kernelI{}EEvNS_10DslashArgsE"
+SM = "_ZN5qmrhs13dslash_kernelI{}EEvNS_10DslashArgsE"
+F32, F64 = "15HIP_vector_typeIfLj2EE", "15HIP_vector_typeIdLj2EE"
+
+# (what runs it, kernel symbol, VGPR budget)
+BUDGETS = [
+    ("FP32 24^3x48 k=16 hypercube R=2", HC.format(f"Li3ELi2ELi0E{F32}S2_S2_fLi16E"), 184),
+    ("FP16 24^3x48 k=16 hypercube R=4", HC.format("Li3ELi4ELi0ENS_3c16ES1_S1_fLi16E"), 248),
+    ("BF16 24^3x48 k=16 hypercube R=2", HC.format("Li3ELi2ELi0ENS_4cb16ES1_S1_fLi16E"), 168),
+    ("FP64 24^3x48 k=16 hypercube R=1", HC.format(f"Li3ELi1ELi0E{F64}S2_S2_dLi16E"), 208),
+    ("FP8 spinors 24^3x48 / 32^3x64 hypercube R=2", HC.format("Li3ELi2ELi0ENS_2c8ENS_3c16ES1_fLi16E"), 152),
+    ("FP16 CG operator 32^3x64 site-major R=4", SM.format("Li3ELi4ELi0ELi0ELb1ELb0ENS_3c16ES1_S1_fLi16E"), 224),
+    ("FP16-in FP32-out CG operator 32^3x64 R=4", SM.format(f"Li3ELi4ELi0ELi0ELb1ELb0ENS_3c16ES1_{F32}fLi16E"), 232),
+    ("FP32 32^3x64 site-major R=2", SM.format(f"Li3ELi2ELi0ELi0ELb1ELb0E{F32}S2_S2_fLi16E"), 184),
+    ("FP64 CG operator 32^3x64 site-major R=1", SM.format(f"Li3ELi1ELi0ELi0ELb1ELb0E{F64}S2_S2_dLi16E"), 200),
+]
+
+
+@pytest.fixture(scope="module")
+def resources():
+    if not os.path.exists(LIB):
+        pytest.skip("libqcu.so not built")
+    return kernel_meta.kernel_resources(LIB)
+
+
+def test_metadata_read(resources):
+    # every translation unit's code object was found (hundreds of kernels)
+    assert len(resources) > 300
+    assert all("vgpr_count" in v for v in resources.values())
+
+
+@pytest.mark.parametrize("what,sym,budget", BUDGETS, ids=[b[0] for b in BUDGETS])
+def test_pinned_kernel_register_budget(resources, what, sym, budget):
+    assert sym in resources, f"{what}: kernel {sym} not in the library"
+    v = resources[sym]
+    used = v["vgpr_count"] + v.get("agpr_count", 0)
+    assert used <= budget, f"{what}: {used} VGPRs against a budget of {budget}"
+    assert v["private_segment_fixed_size"] == 0, f"{what}: {v['private_segment_fixed_size']} B of scratch"
+
+
+def test_no_scratch_in_operator_kernels(resources):
+    """No dslash, hypercube, batch or face kernel for N <= 3 spills to
+    scratch (the FP64 two-RHS-per-lane tiles at N >= 4, the N >= 5 pointer
+    kernels and the clover set-up kernels may)."""
+    bad = []
+    for sym, v in resources.items():
+        m = re.match(r"_ZN5qmrhs(?:12_GLOBAL__N_1)?\d+(dslash_kernel|dslash_hc_kernel|dslash_faces_kernel|"
+                     r"pack_faces_kernel|batch_dslash_kernel)(?:ILi(\d+)E)?", sym)
+        if not m:
+            continue
+        n = int(m.group(2)) if m.group(2) else 0
+        if n <= 3 and v["private_segment_fixed_size"] > 0:
+            bad.append((sym, v["private_segment_fixed_size"]))
+    assert not bad, bad
