@@ -54,7 +54,9 @@ typedef struct qcuMrhsTile_s {
                        bit 2: keep the hardware workgroup order (no XCD-contiguous remap);
                        bit 3 (y-streamed traversals only): XCD-cyclic order, the 8 XCDs work side by side
                        on adjacent halves of y-streamed blocks (ignored when the lattice does
-                       not split into whole chunks of workgroups) */
+                       not split into whole chunks of workgroups);
+                       bit 4 (user AoS layout): each lane reads and writes its own RHS's
+                       records instead of the site's lanes reading them together */
   int occupancy;    /* cap on resident workgroups per CU (3..8), 0 = no cap */
   int traversal;    /* bit 0 = 0: each (t_block, z_block) block in (t, z, y, x) order;
                        bit 0 = 1: the block streamed along y, (y, t, z, x) order inside it;

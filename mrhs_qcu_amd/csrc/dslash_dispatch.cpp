@@ -441,6 +441,52 @@ int launch_dslash(const DslashCall &c, std::string *err)
         shm = lds_bytes;
       }
     }
+    // Group reads and writes of the AoS records (SL_AOS_COOP: a site's lanes
+    // load its neighbours' records together and exchange them in LDS) for a
+    // power-of-two RHS count (a site's lanes inside one wavefront), 16-byte
+    // aligned buffers, no xpay operand and whole 16-byte pieces per record,
+    // with the exchange area beside the staged links (the workgroup halved
+    // until it fits), where they were measured faster at 24^3x48: from 8 RHS
+    // (FP32 N = 2 / 3 / 4 at 16 RHS: 420 -> 276, 671 -> 479, 869 -> 781 us;
+    // FP16: 198 -> 180, 336 -> 289, 416 -> 407 us), from 16 RHS for FP16 at
+    // N >= 4 (8 RHS: 217 -> 227 us); at 4 RHS about even. Instantiated for
+    // FP32 and FP16. Cache hint bit 4 keeps each lane reading its own record.
+    const bool coop_pays = kk >= 8 && !(c.prec_s == PREC_HALF && c.geom.N >= 4 && kk < 16);
+    if (c.sl == SL_AOS_PTRS && f == fn_lds && !(kk & (kk - 1)) && coop_pays && b.aos_vec && !b.xptr &&
+        !(c.tile.hints & 16)) {
+      KernelKey kc = key_lds;
+      kc.sl = SL_AOS_COOP;
+      LaunchFn fc = find_kernel(kc);
+      const size_t rec = size_t(4 * c.geom.N) * size_t(bytes_of(c.prec_s));
+      auto need = [&](int bl, size_t *off) {
+        const size_t links = size_t((bl - 1) / kk + 2) * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
+        *off = (links + 15) / 16 * 16;
+        return *off + size_t(bl) * rec;
+      };
+      int Bc = B;
+      size_t off = 0;
+      while (Bc > 64 && need(Bc, &off) > size_t(kMaxDynLds)) Bc /= 2;
+      const size_t tot = need(Bc, &off);
+      if (fc && rec % 16 == 0 && Bc % kk == 0 && tot <= size_t(kMaxDynLds)) {
+        f = fc;
+        shm = tot;
+        b.coop_off = uint32_t(off);
+        if (Bc != B) {
+          Bl = Bc;
+          b.nblocks = uint32_t((b.nlanes + uint32_t(Bc) - 1) / uint32_t(Bc));
+          if (b.xcd_gran) {
+            Tile t;
+            t.R = 1;
+            t.block = Bc;
+            t.tblock = tb;
+            t.zblock = zb;
+            t.hints = 8;
+            t.ystream = 1;
+            b.xcd_gran = xcd_cycle_gran(c.geom, t, kk);
+          }
+        }
+      }
+    }
     // A hypercube tile (the caller's, from the user-layout entry of the tile
     // table) runs the AoS pointers through the hypercube kernel when this
     // chunk's RHS count is a power of two, the workgroup holds whole blocks

@@ -29,6 +29,8 @@ enum SpinorLayout : int {
   SL_PACKED = 0,    // native: [S][4][N][k] complex, RHS innermost
   SL_AOS_PTRS = 1,  // one pointer per RHS, each the user layout [S][4][N]
   SL_SOA_PTRS = 2,  // one pointer per RHS, each the reference SoA layout [4*N][S]
+  SL_AOS_COOP = 3,  // the user AoS layout of SL_AOS_PTRS read and written by site groups
+                    // (coop_issue / coop_finish / coop_store): a kernel variant, not a format
 };
 enum GaugeLayout : int {
   GL_NATIVE = 0,   // [2][S][8][N][N]: per site 4 forward links then 4 daggered backward links
@@ -93,6 +95,8 @@ struct DslashArgs {
   uint32_t efac_off;  // > 0 (LDS kernels): byte offset in dynamic LDS of the staged per-RHS epilogue
                       // factors [4][k] in the accumulation type (xb sc_in, xa sc_x, 1 / sc_out, rx_alpha)
   uint32_t xcd_gran;  // > 0: XCD-cyclic workgroup order in chunks of xcd_gran workgroups (xcd_cyclic)
+  uint32_t coop_off;  // SL_AOS_COOP: byte offset in dynamic LDS of the group exchange area (16 B
+                      // per piece, blockDim.x * pieces-per-record pieces)
   // Hypercube kernels (dslash_hcube.h): the launch walks a lattice of
   // 2x2x2x2 blocks (subLx x Ly/2 x Lz/2 x Lt/2) in the traversal order above,
   // with tblock / zblock counted in blocks; 2^hc_lp lanes per site.
@@ -547,6 +551,131 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
       sweep<D + 1, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_cur, psi_cur, V_nxt);
     else
       sweep<D + 1, N, R, SL, GL, CS, CG, A, LL>(acc, a, g, slot, blk8, s, psi_nxt, V_nxt, psi_cur, V_cur);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// User AoS pointers read by site groups (SL_AOS_COOP). The P = k lanes of a
+// site (one RHS each; P a power of two, so a group never straddles a
+// wavefront) load the P records of a neighbour site together: lane `slot`
+// loads the 16-byte pieces m = slot + P q, q < Q (Q pieces per record), i.e.
+// piece m % Q of RHS m / Q's record. One load instruction of the wavefront
+// then covers whole records of a few buffers at the wavefront's consecutive
+// sites (~10 cache lines for 16 FP32 RHS) where a lane reading its own record
+// touches ~48 (every lane a different buffer, 16 bytes of each 96-byte
+// record). The pieces are exchanged through the group's LDS area of P Q
+// pieces and every lane reads its own record. Every lane of a group takes part
+// in each exchange (the group's branch conditions are uniform over it).
+template <int N, class CS>
+struct AosRec {
+  static constexpr int Q = 4 * N * int(sizeof(CS)) / 16;  // 16-byte pieces per record
+  static constexpr int C = 16 / int(sizeof(CS));          // complex numbers per piece
+  static_assert((4 * N * int(sizeof(CS))) % 16 == 0, "records in whole 16-byte pieces");
+};
+// The Q pieces a lane holds between issue and exchange.
+template <int N, class CS>
+using AosPieces = u32x4[AosRec<N, CS>::Q];
+
+// Issue the group's loads of the neighbour record at `site` (no wait).
+template <int N, class CS>
+__device__ __forceinline__ void coop_issue(AosPieces<N, CS> &v, const DslashArgs &a, int site, int slot)
+{
+  constexpr int Q = AosRec<N, CS>::Q;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = slot + a.P * q;
+    const int r = m / Q;
+    v[q] = static_cast<const u32x4 *>(a.in_ptrs.p[r])[size_t(site) * Q + (m - r * Q)];
+  }
+}
+
+// Exchange the issued pieces through the group's LDS area `grp` and convert
+// this lane's record into psi.
+template <int N, class CS, class A>
+__device__ __forceinline__ void coop_finish(cpx<A> (&psi)[1][4][N], const AosPieces<N, CS> &v, u32x4 *grp,
+                                            int slot, int P)
+{
+  constexpr int Q = AosRec<N, CS>::Q, C = AosRec<N, CS>::C;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) grp[slot + P * q] = v[q];
+  // The group is inside one wavefront: LDS accesses of a wavefront complete
+  // in order, so only the compiler's order needs holding.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const CVec<CS, C> w = __builtin_bit_cast(CVec<CS, C>, grp[slot * Q + q]);
+#pragma unroll
+    for (int j = 0; j < C; ++j) psi[0][(q * C + j) / N][(q * C + j) % N] = ld_cvt<A>(w.v[j]);
+  }
+  __builtin_amdgcn_wave_barrier();  // the next exchange's stores stay after these loads
+}
+
+// Output of the group: each lane's record (times fin) into the LDS area,
+// then lane `slot` stores pieces m = slot + P q to RHS m / Q at `site`.
+template <int N, class CO, class A>
+__device__ __forceinline__ void coop_store(const DslashArgs &a, const cpx<A> (&acc)[1][4][N], int site, int slot,
+                                           u32x4 *grp, A fin, A &nrm)
+{
+  constexpr int Q = AosRec<N, CO>::Q, C = AosRec<N, CO>::C;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    CVec<CO, C> o;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      cpx<A> v = acc[0][(q * C + j) / N][(q * C + j) % N];
+      v.re *= fin;
+      v.im *= fin;
+      nrm += v.re * v.re + v.im * v.im;
+      o.v[j] = StCvt<CO>::go(v);
+    }
+    grp[slot * Q + q] = __builtin_bit_cast(u32x4, o);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = slot + a.P * q;
+    const int r = m / Q;
+    static_cast<u32x4 *>(a.out_ptrs.p[r])[size_t(site) * Q + (m - r * Q)] = grp[m];
+  }
+}
+
+// The sweep of the group-read kernel: direction D + 1's loads are issued
+// before direction D's arithmetic and exchanged after it. X- comes from the
+// lane holding the same RHS of the site before in the row (its X+), loaded
+// by the groups at the start of a row (g.xown, uniform over a group). Links
+// from LDS.
+template <int D, int N, class CS, class CG, class A>
+__device__ __forceinline__ void sweep_coop(cpx<A> (&acc)[1][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
+                                           const CG *blk8, A s, cpx<A> (&psi_cur)[1][4][N],
+                                           cpx<A> (&psi_nxt)[1][4][N], u32x4 *grp)
+{
+  AosPieces<N, CS> v;
+  if constexpr (D == 0) {
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+        psi_nxt[0][sp][c] = cpx<A>{lane_read(psi_cur[0][sp][c].re, g.xsrc), lane_read(psi_cur[0][sp][c].im, g.xsrc)};
+    if (g.xown) coop_issue<N, CS>(v, a, g.nbr[1], slot);
+  } else if constexpr (D + 1 < 8) {
+    coop_issue<N, CS>(v, a, g.nbr[D + 1], slot);
+  }
+  if (g.live[D]) {
+    cpx<A> V[N][N];
+    fetch_link<D, N, GL_NATIVE, CG, A>(V, a, g, blk8);
+    hop_one<N, D, A>(acc[0], V, psi_cur[0], s);
+  }
+  if constexpr (D + 1 < 8) {
+    if constexpr (D == 0) {
+      if (g.xown) coop_finish<N, CS, A>(psi_nxt, v, grp, slot, a.P);
+    } else {
+      coop_finish<N, CS, A>(psi_nxt, v, grp, slot, a.P);
+    }
+    sweep_coop<D + 1, N, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, psi_cur, grp);
   }
 }
 
@@ -1173,7 +1302,19 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
 #pragma unroll
         for (int c = 0; c < N; ++c) acc[r][sp][c] = cpx<A>{A(-0.0), A(-0.0)};  // x + (-0) == x: folds
 
-    if constexpr (SL == SL_PACKED && site_blocked(GL) && std::is_same<CS, CG>::value &&
+    u32x4 *grp = nullptr;  // SL_AOS_COOP: this site group's LDS exchange area
+    if constexpr (SL == SL_AOS_COOP) {
+      static_assert(GL == GL_NATIVE && LDS && !SITE_MINOR && R == 1 && N < kRowLinkN,
+                    "group-read AoS kernels: native links staged in LDS, site-major lanes, one RHS per lane");
+      grp = reinterpret_cast<u32x4 *>(smem + a.coop_off) + size_t(threadIdx.x - slot) * AosRec<N, CS>::Q;
+      cpx<A> psi0[1][4][N], psi1[1][4][N];
+      {
+        AosPieces<N, CS> v;
+        coop_issue<N, CS>(v, a, g.nbr[0], int(slot));
+        coop_finish<N, CS, A>(psi0, v, grp, int(slot), a.P);
+      }
+      sweep_coop<0, N, CS, CG, A>(acc, a, g, int(slot), blk8, s, psi0, psi1, grp);
+    } else if constexpr (SL == SL_PACKED && site_blocked(GL) && std::is_same<CS, CG>::value &&
                   (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) && std::is_same<A, float>::value) {
       using PV = typename Pair<CS>::V;
       PV psi0[R][4][N], psi1[R][4][N], V0[N][N], V1[N][N];
@@ -1194,6 +1335,9 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
       if (a.efac_off) ef = reinterpret_cast<const A *>(smem + a.efac_off);
     if constexpr (SL == SL_PACKED) {
       epilogue_packed<N, R, CS, CO, A>(a, acc, g.idx, slot, ef, nrm);
+    } else if constexpr (SL == SL_AOS_COOP) {
+      const A fin = A(a.xb) * (a.sc_in ? A(a.sc_in[slot]) : A(1));
+      coop_store<N, CO, A>(a, acc, g.idx, int(slot), grp, fin, nrm[0]);
     } else {
       epilogue_ptrs<N, SL, CS, CO, A>(a, acc, g.idx, slot, nrm);
     }

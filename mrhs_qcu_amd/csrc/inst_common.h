@@ -23,6 +23,11 @@
   QM_REG(N, 1, ::qmrhs::SL_PACKED, ::qmrhs::GL_NATIVE, true, false, CS, CG, CO, A)
 
 // Pointer layouts (one RHS per lane).
+// User AoS pointers read and written by site groups (N <= 4, links in LDS;
+// FP32 and FP16: in FP64 the held pieces cost the kernel a wave per SIMD).
+#define QM_PTRS_COOP(N, CS, A) \
+  QM_REG(N, 1, ::qmrhs::SL_AOS_COOP, ::qmrhs::GL_NATIVE, true, false, CS, CS, CS, A)
+
 #define QM_PTRS(N, CS, A)                                                             \
   QM_REG(N, 1, ::qmrhs::SL_AOS_PTRS, ::qmrhs::GL_NATIVE, false, false, CS, CS, CS, A) \
   QM_REG(N, 1, ::qmrhs::SL_AOS_PTRS, ::qmrhs::GL_NATIVE, true, false, CS, CS, CS, A)  \
