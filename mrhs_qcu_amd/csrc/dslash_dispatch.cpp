@@ -391,7 +391,12 @@ int launch_dslash(const DslashCall &c, std::string *err)
     // XCDs side by side on adjacent halves (24^3x48 FP32, 16 RHS: 720 ->
     // 658 us); the natural order when the lattice does not divide.
     int tb = 1, zb = b.Lz;
-    if (c.sl == SL_AOS_PTRS) {
+    if (c.sl == SL_AOS_PTRS && c.tile.ystream && !c.tile.hcube && c.tile.tblock > 1 && c.tile.zblock > 0 &&
+        b.Lt % c.tile.tblock == 0 && b.Lz % c.tile.zblock == 0) {
+      // A y-streamed user-layout tile gives the blocking itself.
+      tb = c.tile.tblock;
+      zb = c.tile.zblock;
+    } else if (c.sl == SL_AOS_PTRS) {
       for (int t : {3, 2, 4, 6})
         if (b.Lt % t == 0 && t < b.Lt) {
           tb = t;
@@ -458,10 +463,11 @@ int launch_dslash(const DslashCall &c, std::string *err)
       kc.sl = SL_AOS_COOP;
       LaunchFn fc = find_kernel(kc);
       const size_t rec = size_t(4 * c.geom.N) * size_t(bytes_of(c.prec_s));
+      const size_t slot_bytes = (rec / 16 | 1) * 16;  // a record's slot of the exchange area (AosRec::QS)
       auto need = [&](int bl, size_t *off) {
         const size_t links = size_t((bl - 1) / kk + 2) * 8 * c.geom.N * c.geom.N * size_t(bytes_of(c.prec_g));
         *off = (links + 15) / 16 * 16;
-        return *off + size_t(bl) * rec;
+        return *off + size_t(bl) * slot_bytes;
       };
       int Bc = B;
       size_t off = 0;

@@ -564,29 +564,50 @@ __device__ __forceinline__ void sweep(cpx<A> (&acc)[R][4][N], const DslashArgs &
 // sites (~10 cache lines for 16 FP32 RHS) where a lane reading its own record
 // touches ~48 (every lane a different buffer, 16 bytes of each 96-byte
 // record). The pieces are exchanged through the group's LDS area of P Q
-// pieces and every lane reads its own record. Every lane of a group takes part
+// pieces (each record in a slot of QS >= Q pieces) and every lane reads its own
+// record. Every lane of a group takes part
 // in each exchange (the group's branch conditions are uniform over it).
 template <int N, class CS>
 struct AosRec {
   static constexpr int Q = 4 * N * int(sizeof(CS)) / 16;  // 16-byte pieces per record
   static constexpr int C = 16 / int(sizeof(CS));          // complex numbers per piece
+  // Pieces per record slot of the LDS exchange area: odd, so that lanes
+  // reading their own records at a stride of QS pieces spread over the banks.
+  static constexpr int QS = Q | 1;
   static_assert((4 * N * int(sizeof(CS))) % 16 == 0, "records in whole 16-byte pieces");
 };
 // The Q pieces a lane holds between issue and exchange.
 template <int N, class CS>
 using AosPieces = u32x4[AosRec<N, CS>::Q];
 
-// Issue the group's loads of the neighbour record at `site` (no wait).
+// The base of each piece this lane loads: piece m = slot + P q is piece
+// m % Q of RHS m / Q's record (read from the argument block once, not once
+// per direction).
 template <int N, class CS>
-__device__ __forceinline__ void coop_issue(AosPieces<N, CS> &v, const DslashArgs &a, int site, int slot)
+struct AosBases {
+  const u32x4 *p[AosRec<N, CS>::Q];
+};
+template <int N, class CS>
+__device__ __forceinline__ AosBases<N, CS> coop_bases(const DslashArgs &a, int slot)
 {
   constexpr int Q = AosRec<N, CS>::Q;
+  AosBases<N, CS> b;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = slot + a.P * q;
     const int r = m / Q;
-    v[q] = static_cast<const u32x4 *>(a.in_ptrs.p[r])[size_t(site) * Q + (m - r * Q)];
+    b.p[q] = static_cast<const u32x4 *>(a.in_ptrs.p[r]) + (m - r * Q);
   }
+  return b;
+}
+
+// Issue the group's loads of the neighbour record at `site` (no wait).
+template <int N, class CS>
+__device__ __forceinline__ void coop_issue(AosPieces<N, CS> &v, const AosBases<N, CS> &b, int site)
+{
+  constexpr int Q = AosRec<N, CS>::Q;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = b.p[q][size_t(site) * Q];
 }
 
 // Exchange the issued pieces through the group's LDS area `grp` and convert
@@ -595,9 +616,13 @@ template <int N, class CS, class A>
 __device__ __forceinline__ void coop_finish(cpx<A> (&psi)[1][4][N], const AosPieces<N, CS> &v, u32x4 *grp,
                                             int slot, int P)
 {
-  constexpr int Q = AosRec<N, CS>::Q, C = AosRec<N, CS>::C;
+  constexpr int Q = AosRec<N, CS>::Q, C = AosRec<N, CS>::C, QS = AosRec<N, CS>::QS;
 #pragma unroll
-  for (int q = 0; q < Q; ++q) grp[slot + P * q] = v[q];
+  for (int q = 0; q < Q; ++q) {
+    const int m = slot + P * q;
+    const int r = m / Q;
+    grp[r * QS + (m - r * Q)] = v[q];
+  }
   // The group is inside one wavefront: LDS accesses of a wavefront complete
   // in order, so only the compiler's order needs holding.
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -605,7 +630,7 @@ __device__ __forceinline__ void coop_finish(cpx<A> (&psi)[1][4][N], const AosPie
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const CVec<CS, C> w = __builtin_bit_cast(CVec<CS, C>, grp[slot * Q + q]);
+    const CVec<CS, C> w = __builtin_bit_cast(CVec<CS, C>, grp[slot * QS + q]);
 #pragma unroll
     for (int j = 0; j < C; ++j) psi[0][(q * C + j) / N][(q * C + j) % N] = ld_cvt<A>(w.v[j]);
   }
@@ -618,7 +643,7 @@ template <int N, class CO, class A>
 __device__ __forceinline__ void coop_store(const DslashArgs &a, const cpx<A> (&acc)[1][4][N], int site, int slot,
                                            u32x4 *grp, A fin, A &nrm)
 {
-  constexpr int Q = AosRec<N, CO>::Q, C = AosRec<N, CO>::C;
+  constexpr int Q = AosRec<N, CO>::Q, C = AosRec<N, CO>::C, QS = AosRec<N, CO>::QS;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     CVec<CO, C> o;
@@ -630,7 +655,7 @@ __device__ __forceinline__ void coop_store(const DslashArgs &a, const cpx<A> (&a
       nrm += v.re * v.re + v.im * v.im;
       o.v[j] = StCvt<CO>::go(v);
     }
-    grp[slot * Q + q] = __builtin_bit_cast(u32x4, o);
+    grp[slot * QS + q] = __builtin_bit_cast(u32x4, o);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -639,7 +664,7 @@ __device__ __forceinline__ void coop_store(const DslashArgs &a, const cpx<A> (&a
   for (int q = 0; q < Q; ++q) {
     const int m = slot + a.P * q;
     const int r = m / Q;
-    static_cast<u32x4 *>(a.out_ptrs.p[r])[size_t(site) * Q + (m - r * Q)] = grp[m];
+    static_cast<u32x4 *>(a.out_ptrs.p[r])[size_t(site) * Q + (m - r * Q)] = grp[r * QS + (m - r * Q)];
   }
 }
 
@@ -651,7 +676,7 @@ __device__ __forceinline__ void coop_store(const DslashArgs &a, const cpx<A> (&a
 template <int D, int N, class CS, class CG, class A>
 __device__ __forceinline__ void sweep_coop(cpx<A> (&acc)[1][4][N], const DslashArgs &a, const SiteGeo &g, int slot,
                                            const CG *blk8, A s, cpx<A> (&psi_cur)[1][4][N],
-                                           cpx<A> (&psi_nxt)[1][4][N], u32x4 *grp)
+                                           cpx<A> (&psi_nxt)[1][4][N], u32x4 *grp, const AosBases<N, CS> &bp)
 {
   AosPieces<N, CS> v;
   if constexpr (D == 0) {
@@ -660,9 +685,9 @@ __device__ __forceinline__ void sweep_coop(cpx<A> (&acc)[1][4][N], const DslashA
 #pragma unroll
       for (int c = 0; c < N; ++c)
         psi_nxt[0][sp][c] = cpx<A>{lane_read(psi_cur[0][sp][c].re, g.xsrc), lane_read(psi_cur[0][sp][c].im, g.xsrc)};
-    if (g.xown) coop_issue<N, CS>(v, a, g.nbr[1], slot);
+    if (g.xown) coop_issue<N, CS>(v, bp, g.nbr[1]);
   } else if constexpr (D + 1 < 8) {
-    coop_issue<N, CS>(v, a, g.nbr[D + 1], slot);
+    coop_issue<N, CS>(v, bp, g.nbr[D + 1]);
   }
   if (g.live[D]) {
     cpx<A> V[N][N];
@@ -675,7 +700,7 @@ __device__ __forceinline__ void sweep_coop(cpx<A> (&acc)[1][4][N], const DslashA
     } else {
       coop_finish<N, CS, A>(psi_nxt, v, grp, slot, a.P);
     }
-    sweep_coop<D + 1, N, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, psi_cur, grp);
+    sweep_coop<D + 1, N, CS, CG, A>(acc, a, g, slot, blk8, s, psi_nxt, psi_cur, grp, bp);
   }
 }
 
@@ -1306,14 +1331,15 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
     if constexpr (SL == SL_AOS_COOP) {
       static_assert(GL == GL_NATIVE && LDS && !SITE_MINOR && R == 1 && N < kRowLinkN,
                     "group-read AoS kernels: native links staged in LDS, site-major lanes, one RHS per lane");
-      grp = reinterpret_cast<u32x4 *>(smem + a.coop_off) + size_t(threadIdx.x - slot) * AosRec<N, CS>::Q;
+      grp = reinterpret_cast<u32x4 *>(smem + a.coop_off) + size_t(threadIdx.x - slot) * AosRec<N, CS>::QS;
       cpx<A> psi0[1][4][N], psi1[1][4][N];
+      const AosBases<N, CS> bp = coop_bases<N, CS>(a, int(slot));
       {
         AosPieces<N, CS> v;
-        coop_issue<N, CS>(v, a, g.nbr[0], int(slot));
+        coop_issue<N, CS>(v, bp, g.nbr[0]);
         coop_finish<N, CS, A>(psi0, v, grp, int(slot), a.P);
       }
-      sweep_coop<0, N, CS, CG, A>(acc, a, g, int(slot), blk8, s, psi0, psi1, grp);
+      sweep_coop<0, N, CS, CG, A>(acc, a, g, int(slot), blk8, s, psi0, psi1, grp, bp);
     } else if constexpr (SL == SL_PACKED && site_blocked(GL) && std::is_same<CS, CG>::value &&
                   (std::is_same<CS, c16>::value || std::is_same<CS, cb16>::value) && std::is_same<A, float>::value) {
       using PV = typename Pair<CS>::V;
