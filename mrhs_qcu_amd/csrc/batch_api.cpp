@@ -124,6 +124,14 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       t.block = kBatchBlock;
       t.occ = 0;
       t.hcube = 0;  // the work-list body is the site-major kernel (the blocking still divides the lattice)
+      if (t.trav_hi) {
+        // A column-order tile's blocking is in (y, z) blocks: default traversal instead.
+        t.trav_hi = 0;
+        t.ystream = 0;
+        t.tblock = 1;
+        t.zblock = 0;
+        t.hints = 0;
+      }
     }
   }
   if (!worklist)

@@ -251,6 +251,32 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
     a.hdiv_row = make_fastdiv(uint32_t(a.subLx * a.bLy));
     a.hdiv_nzb = make_fastdiv(uint32_t(bz / a.zblock));
     a.hc_lp = uint32_t(__builtin_ctz(unsigned(P)));
+    if (c.tile.trav_hi == 1) {
+      // Column order: t_block / z_block are the column's extent in y / z
+      // blocks; a column's slab at one T is a whole number of workgroups.
+      const int cy = c.tile.tblock, cz = c.tile.zblock;
+      const int slab = a.subLx * cy * cz;
+      const int wpb = B / (8 * P);  // blocks per workgroup
+      if (slab % wpb) return fail(err, "column order: a column's slab must fill whole workgroups"), 1;
+      const uint32_t ncol = uint32_t((a.bLy / cy) * (bz / cz));
+      const uint32_t nT = uint32_t(c.geom.L[3] / 2);
+      const uint32_t ws = uint32_t(slab / wpb);
+      a.hc_col = 1;
+      a.col_cy = uint32_t(cy);
+      a.col_cz = uint32_t(cz);
+      a.col_ncol = ncol;
+      a.col_ncy = uint32_t(a.bLy / cy);
+      a.col_nT = nT;
+      a.col_slab = uint32_t(slab);
+      a.col_grp = nT * ws;
+      a.cdiv_slab = make_fastdiv(a.col_slab);
+      a.cdiv_nT = make_fastdiv(nT);
+      a.cdiv_cy = make_fastdiv(a.col_cy);
+      a.cdiv_ncy = make_fastdiv(a.col_ncy);
+      a.cdiv_grp = make_fastdiv(a.col_grp);
+    }
+  } else if (c.tile.trav_hi) {
+    return fail(err, "column order needs a hypercube kernel for these precisions / N / rhs_per_lane"), 1;
   }
   KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
   key.hc = hc;
@@ -268,9 +294,18 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   if (lanes >= (1ULL << 32)) return fail(err, "too many lanes for one launch"), 1;
   a.nlanes = uint32_t(lanes);
   a.nblocks = uint32_t((lanes + B - 1) / B);
+  a.ngrid = a.nblocks;
   Tile tl = c.tile;
   tl.hcube = hc;
-  a.xcd_gran = xcd_cycle_gran(c.geom, tl, c.k);
+  a.xcd_gran = a.hc_col ? 0u : xcd_cycle_gran(c.geom, tl, c.k);
+  if (a.hc_col) {
+    // Every logical workgroup belongs to one column slab (lanes divide into
+    // whole workgroups), and the grid pads the last group of 8 columns.
+    if (lanes % uint64_t(B) || uint64_t(a.nblocks) != uint64_t(a.col_ncol) * a.col_grp)
+      return fail(err, "column order: the columns do not tile the lattice"), 1;
+    a.ngrid = (a.col_ncol + 7u) / 8u * 8u * a.col_grp;
+    a.xcd_map = 0;
+  }
   size_t shm = 0;
   if (lds) {
     const size_t nsite_max = size_t((B - 1) / a.P + 2);
@@ -312,8 +347,17 @@ static int check_call(const DslashCall &c, std::string *err)
                      "sides and tiles with rhs_per_lane 2 or 4 and LDS-staged links"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
-  if (c.tile.trav_hi) return fail(err, "traversal bits 2 and above are not defined"), 1;
-  if (!c.tile.hcube) {
+  if (c.tile.trav_hi > 1) return fail(err, "traversal bits 3 and above are not defined"), 1;
+  if (c.tile.trav_hi) {
+    // Column order (hypercube, packed layout): t_block x z_block blocks in (y, z).
+    if (!c.tile.hcube || c.tile.ystream || c.sl != SL_PACKED)
+      return fail(err, "column order (traversal bit 2) is a hypercube traversal of the packed layout "
+                       "(traversal 6)"), 1;
+    if (c.tile.tblock < 1 || (c.geom.L[1] / 2) % c.tile.tblock)
+      return fail(err, "column order: t_block (column extent in y blocks) must divide Ly / 2"), 1;
+    if (c.tile.zblock < 1 || (c.geom.L[2] / 2) % c.tile.zblock)
+      return fail(err, "column order: z_block (column extent in z blocks) must divide Lz / 2"), 1;
+  } else if (!c.tile.hcube) {
     if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
     if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
   } else {
@@ -329,8 +373,26 @@ static int check_call(const DslashCall &c, std::string *err)
   return 0;
 }
 
-int packed_launch_args(const DslashCall &c, DslashArgs *a, size_t *shm, std::string *err)
+// A column-order tile (traversal bit 2) on a call that no hypercube kernel
+// serves (compressed links, pointer layouts, precisions / N / rhs_per_lane
+// without one) runs with the default traversal of its rhs_per_lane, block
+// and link staging instead.
+static DslashCall effective_call(const DslashCall &c0)
 {
+  if (!c0.tile.trav_hi || runs_hcube(c0)) return c0;
+  DslashCall c = c0;
+  c.tile.trav_hi = 0;
+  c.tile.hcube = 0;
+  c.tile.ystream = 0;
+  c.tile.tblock = 1;
+  c.tile.zblock = 0;
+  c.tile.hints = 0;
+  return c;
+}
+
+int packed_launch_args(const DslashCall &c0, DslashArgs *a, size_t *shm, std::string *err)
+{
+  const DslashCall c = effective_call(c0);
   if (c.sl != SL_PACKED) return fail(err, "work lists hold packed-layout launches"), 1;
   if (check_call(c, err)) return 1;
   *a = DslashArgs{};
@@ -339,8 +401,9 @@ int packed_launch_args(const DslashCall &c, DslashArgs *a, size_t *shm, std::str
   return prepare_packed(c, a, shm, &fn, err);
 }
 
-int launch_dslash(const DslashCall &c, std::string *err)
+int launch_dslash(const DslashCall &c0, std::string *err)
 {
+  const DslashCall c = effective_call(c0);
   if (check_call(c, err)) return 1;
   const int B = c.tile.block;
 
@@ -351,7 +414,7 @@ int launch_dslash(const DslashCall &c, std::string *err)
     size_t shm = 0;
     LaunchFn fn = nullptr;
     if (prepare_packed(c, &a, &shm, &fn, err)) return 1;
-    hipError_t e = fn(a, dim3(a.nblocks), dim3(B), shm, c.stream);
+    hipError_t e = fn(a, dim3(a.ngrid), dim3(B), shm, c.stream);
     if (e != hipSuccess) return fail(err, std::string("launch failed: ") + hipGetErrorString(e)), 1;
     return 0;
   }

@@ -108,8 +108,12 @@ struct Tile {
   int ystream = 0; // 1: stream each (tblock, zblock) block along y
   int hcube = 0;   // 1: hypercube work mapping (2x2x2x2 blocks, one per 8 P lanes; tblock / zblock
                    // then count blocks)
-  int trav_hi = 0; // traversal bits 2 and above: none defined (a tile that sets them is rejected)
+  int trav_hi = 0; // traversal bits 2 and above: 1 = column order of the hypercube mapping (tblock /
+                   // zblock = a column's extent in y / z blocks; XCD x walks columns 8 g + x along t)
 };
+
+// The traversal field of qcuMrhsTile: bit 0 ystream, bit 1 hcube, bit 2 column order.
+inline int tile_traversal(const Tile &t) { return t.ystream | (t.hcube << 1) | (t.trav_hi << 2); }
 
 struct DslashCall {
   int sl = SL_PACKED, gl = GL_NATIVE;

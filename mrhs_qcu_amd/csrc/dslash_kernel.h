@@ -103,6 +103,17 @@ struct DslashArgs {
   uint32_t hc_lp;
   int bLy, bLz;
   FastDiv hdiv_Ly, hdiv_row, hdiv_nzb;
+  // Column order of the hypercube kernels (traversal bit 2): the block
+  // lattice is cut into columns of col_cy x col_cz blocks in (y, z), whole in
+  // x and t; the logical block number is ((column * nT + T) * slab + b) with
+  // b = (z_in * col_cy + y_in) * subLx + xh inside the column's slab at T.
+  // XCD x walks column 8 g + x slab by slab from T = 0 to nT - 1, for
+  // g = 0, 1, ... (col_remap); the launch has 8 ceil(col_ncol / 8) col_grp
+  // workgroups, the ones past the last column idle.
+  int hc_col;
+  uint32_t col_cy, col_cz, col_ncol, col_ncy, col_nT, col_slab, col_grp;
+  FastDiv cdiv_slab, cdiv_nT, cdiv_cy, cdiv_ncy, cdiv_grp;
+  uint32_t ngrid;  // workgroups launched: nblocks, or the padded column-order grid
   // Pointer layouts only. Kept last: the packed kernels read only the fields
   // above, which then share a few scalar-cache lines of the argument block
   // (at 24^3x48, 16 RHS, FP32 the packed kernel took 4 us longer with these

@@ -130,7 +130,8 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.occ = v[14];
     t.ystream = v[15] & 1;
     t.hcube = (v[15] >> 1) & 1;
-    if (v[15] >> 2) continue;  // traversal bits 2 and above: not defined
+    t.trav_hi = v[15] >> 2;
+    if (t.trav_hi > 1 || (t.trav_hi && (!t.hcube || t.ystream))) continue;  // not a defined traversal
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -144,8 +145,7 @@ void append_tune_file(const TuneKey &k, const Tile &t)
   if (!f) return;
   fprintf(f, "%s %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d\n", device_arch().c_str(), std::get<0>(k),
           std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k),
-          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
-          t.ystream | (t.hcube << 1));
+          std::get<7>(k), t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, tile_traversal(t));
   fclose(f);
 }
 
@@ -305,6 +305,24 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
           if (xcd_cycle_gran(g, t, k)) out.push_back(t);
         }
       }
+      // Column order: columns of cy x cz blocks in (y, z), XCD x walking
+      // columns 8 g + x along t; only column counts that fill every XCD.
+      for (int cy : {1, 2, 3, 4, 6, 8}) {
+        for (int cz : {1, 2, 3, 4, 6, 8}) {
+          const int by = g.L[1] / 2, bz = g.L[2] / 2;
+          if (by % cy || bz % cz || ((by / cy) * (bz / cz)) % 8) continue;
+          if ((g.L[0] / 2 * cy * cz) % (block / (8 * P))) continue;
+          Tile t;
+          t.R = R;
+          t.lds = 1;
+          t.block = block;
+          t.tblock = cy;
+          t.zblock = cz;
+          t.hcube = 1;
+          t.trav_hi = 1;
+          out.push_back(t);
+        }
+      }
     }
   }
   return out;
@@ -336,7 +354,7 @@ qcuMrhsTile tile_to_c(const Tile &t)
   c.z_block = t.zblock;
   c.cache_hints = t.hints;
   c.occupancy = t.occ;
-  c.traversal = t.ystream | (t.hcube << 1);
+  c.traversal = tile_traversal(t);
   return c;
 }
 
@@ -622,7 +640,8 @@ int qcuMrhsSetTile(const qcuMrhsGeom *gc, int prec, int nrhs, const qcuMrhsTile 
   const Tile t = tile_from_c(*tile);
   if (t.R != 1 && t.R != 2 && t.R != 4) return api_fail("rhs_per_lane must be 1, 2 or 4");
   if (t.block < 64 || t.block > 256 || t.block % 64) return api_fail("block must be 64, 128, 192 or 256");
-  if (t.trav_hi) return api_fail("traversal bits 2 and above are not defined");
+  if (t.trav_hi > 1) return api_fail("traversal bits 3 and above are not defined");
+  if (t.trav_hi && (!t.hcube || t.ystream)) return api_fail("column order (traversal bit 2) needs traversal 6");
   set_tile(g, prec, nrhs, t);
   return 0;
 }
@@ -660,8 +679,7 @@ int qcuMrhsTune(void *out, const void *in, const void *gauge, int prec, int nrhs
   auto log_tile = [&](const char *round, const Tile &t, double us) {
     if (log)
       fprintf(stderr, "qcuMrhsTune %s N=%d k=%d prec=%d tile=%d,%d,%d,%d,%d,%d,%d,%d us=%.2f\n", round, g.N, nrhs,
-              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ,
-              t.ystream | (t.hcube << 1), us);
+              prec, t.R, t.block, t.lds, t.tblock, t.zblock, t.hints, t.occ, tile_traversal(t), us);
   };
   for (const Tile &t : cands) {
     const double us = time_tile(t, 5);

@@ -288,14 +288,22 @@ def test_pinned_tile_table():
         assert len(f) == 17 and f[0] == "gfx950", line
         N, k, prec, Lx, Ly, Lz, Lt, mask, R, block, lds, tb, zb, hints, occ, trav = map(int, f[1:])
         assert k % R == 0 and block in (64, 128, 192, 256) and lds in (0, 1), line
-        assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
-        assert trav in (0, 1, 2, 3), line
+        assert trav in (0, 1, 2, 3, 6), line
         if trav & 2:
             # hypercube mapping: blocking counted in 2x2x2x2 blocks, whole
             # blocks of 8 P lanes per workgroup, links staged in LDS
             P = k // R
             assert P & (P - 1) == 0 and P <= 32 and block % (8 * P) == 0 and lds == 1 and N <= 4, line
-            assert (Lt // 2) % max(tb, 1) == 0 and (zb == 0 or (Lz // 2) % zb == 0), line
+        if trav == 6:
+            # column order: t_block x z_block blocks in (y, z); the columns fill
+            # whole workgroups and every XCD (a multiple of 8 columns)
+            by, bz = Ly // 2, Lz // 2
+            assert by % tb == 0 and bz % zb == 0 and (by // tb) * (bz // zb) % 8 == 0, line
+            assert (Lx // 2 * tb * zb) % (block // (8 * (k // R))) == 0, line
+        else:
+            assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
+            if trav & 2:
+                assert (Lt // 2) % max(tb, 1) == 0 and (zb == 0 or (Lz // 2) % zb == 0), line
         shapes.add((N, k, prec, (Lx, Ly, Lz, Lt), mask))
     for L in ((24, 24, 24, 48), (32, 32, 32, 64)):
         for prec in (0, 1, 2, 4, 5):
@@ -319,5 +327,11 @@ def test_set_tile_round_trip(lib):
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 96))) != 0
     assert b"block" in lib.qcuMrhsLastError()
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 128, 1, 1, 1, 8, 0, 3 | 4))) != 0
-    assert b"traversal bits" in lib.qcuMrhsLastError()
+    assert b"traversal 6" in lib.qcuMrhsLastError()
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 128, 1, 1, 1, 8, 0, 2 | 8))) != 0
+    assert b"traversal bits 3" in lib.qcuMrhsLastError()
+    col = (2, 128, 1, 2, 5, 0, 0, 6)  # column order: 2 x 5 blocks in (y, z)
+    assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 4, ctypes.byref(_lib.tile(*col))) == 0
+    assert lib.qcuMrhsGetTile(g, _lib.SINGLE, 4, ctypes.byref(t)) == 0
+    assert t.as_tuple() == col
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, None) != 0

@@ -401,6 +401,42 @@ def test_qcu_h_many_rhs_chunks(lib, dev):
             assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger)) < TOL[2], (N, parity, dagger)
 
 
+def test_qcu_h_push_buffer_reused_at_once(lib, dev):
+    """pushBackFermions returns only once the input has been read: a caller
+    that refills one input buffer for the next RHS right after each push, on
+    a stream that does not wait for the library's, still gets every RHS's own
+    result. 24^3x48 FP64 (64 MB per RHS, so the push copy takes ~10 us),
+    k = 3, against the oracle."""
+    L, N, k = (24, 24, 24, 48), 3, 3
+    g = fields.gauge_field(L, N, seed=41)
+    psi = fields.gaussian_spinors(L, N, k, seed=42)
+    grid = _lib.QcuGrid()
+    param = _lib.QcuParam()
+    for d in range(4):
+        grid.grid_size[d] = 1
+        param.lattice_size[d] = L[d]
+    lib.initGridSize(ctypes.byref(grid), ctypes.byref(param), N)
+    gd = dev_tensor(g, dev)
+    lib.loadQcuGauge(gd.data_ptr())
+    lib.getDslash(0, 0.0, N, k, 2)
+    srcs = [dev_tensor(psi[r], dev) for r in range(k)]
+    buf = torch.empty_like(srcs[0])
+    outs = [torch.full_like(t, float("nan")) for t in srcs]
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for r in range(k):
+        with torch.cuda.stream(side):
+            buf.copy_(srcs[r])
+        side.synchronize()
+        lib.pushBackFermions(outs[r].data_ptr(), buf.data_ptr())
+    with torch.cuda.stream(side):
+        buf.fill_(float("nan"))
+    side.synchronize()
+    lib.startDslash(1, 0)
+    got = np.stack([host_complex(t) for t in outs])
+    assert rel_err(got, oracle.dslash(g, psi, L, 1, 0)) < TOL[2]
+
+
 @pytest.mark.parametrize("prec", [_lib.SINGLE, _lib.HALF])
 def test_full_size_properties(lib, dev, prec):
     """24^3x48, nrhs = 16: RHS independence (a permutation of the RHS permutes

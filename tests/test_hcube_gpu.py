@@ -19,6 +19,7 @@ from test_dslash_gpu import pack_inputs, unpack, stream, TOL
 pytestmark = pytest.mark.gpu
 
 HC = 2  # traversal bit of the hypercube mapping
+COL = 4  # traversal bit of the column order (with HC: traversal 6)
 
 
 def hc_tiles(L, k, blocks=(64, 128, 192, 256)):
@@ -143,7 +144,10 @@ def test_hcube_rejects_bad_tiles(lib, dev):
     for t, msg in (((1, 64, 1, 1, 0, 0, 0, HC), "multiple of 8 * nrhs"),     # 16 lanes per site
                    ((2, 128, 1, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
                    ((2, 128, 0, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
-                   ((2, 128, 1, 1, 0, 0, 0, HC | 4), "traversal bits 2 and above")):
+                   ((2, 128, 1, 1, 0, 0, 0, HC | 8), "traversal bits 3 and above"),
+                   ((2, 128, 1, 1, 1, 0, 0, HC | COL | 1), "traversal 6"),
+                   ((2, 128, 1, 4, 1, 0, 0, HC | COL), "column order: t_block"),
+                   ((2, 128, 1, 1, 2, 0, 0, HC | COL), "column order: z_block")):
         rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 0, 0, geom,
                                _lib.tile(*t), stream())
         assert rc != 0 and msg in lib.qcuMrhsLastError().decode(), (t, lib.qcuMrhsLastError())
@@ -350,3 +354,111 @@ def test_hcube_fp8(lib, dev):
             print(f"FP8 hypercube tile {t}, parity {parity}: error vs FP64 oracle {err:.3e}, "
                   f"vs site-major FP8 {rel_err(got, ref):.3e}")
             assert err < 0.1 and rel_err(got, ref) < 0.03, (t, parity)
+
+
+def col_tiles(L, k, Rs=(1, 2, 4), blocks=(64, 128, 256)):
+    """Column-order tiles (traversal 6) whose column slabs fill whole workgroups."""
+    by, bz = L[1] // 2, L[2] // 2
+    for R in Rs:
+        P = k // R if k % R == 0 else 0
+        if not P or P > 32 or P & (P - 1):
+            continue
+        for block in blocks:
+            if block % (8 * P):
+                continue
+            for cy in [d for d in (1, 2, 3, 4) if by % d == 0]:
+                for cz in [d for d in (1, 2, 3, 6) if bz % d == 0]:
+                    if (L[0] // 2 * cy * cz) % (block // (8 * P)) == 0:
+                        yield (R, block, 1, cy, cz, 0, 0, HC | COL)
+
+
+def test_column_order_every_tile_fp32(lib, dev):
+    """Column order: XCD x walks columns 8 g + x along t, the last group of 8
+    padded with idle workgroups (1 to 24 columns here). Every column shape,
+    RHS per lane and workgroup size for k = 16, 8, 2 against the oracle."""
+    L, N = (8, 8, 12, 8), 3
+    g = fields.gauge_field(L, N, seed=131)
+    for k in (16, 8, 2):
+        psi = fields.gaussian_spinors(L, N, k, seed=132 + k)
+        want = {pd: oracle.dslash(g, psi, L, *pd) for pd in ((0, 0), (1, 1))}
+        geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+        n = 0
+        for i, t in enumerate(col_tiles(L, k)):
+            parity, dagger = ((0, 0), (1, 1))[i & 1]
+            pout = torch.full_like(pin, float("nan"))
+            rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, parity, dagger,
+                                   geom, _lib.tile(*t), stream())
+            if rc != 0:
+                msg = lib.qcuMrhsLastError().decode()
+                assert "LDS" in msg, (t, msg)
+                continue
+            got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+            err = rel_err(got, want[(parity, dagger)])
+            assert err < TOL[1], (k, t, err)
+            n += 1
+        assert n >= 10, (k, n)
+
+
+@pytest.mark.parametrize("prec,k,R", [(_lib.DOUBLE, 4, 1), (_lib.HALF, 16, 4), (_lib.BF16, 16, 2),
+                                      (_lib.HALF, 8, 2)])
+def test_column_order_precisions(lib, dev, prec, k, R):
+    """Column order with FP64, FP16 and BF16 storage, boundary skips on a
+    split lattice included."""
+    L, N = (8, 4, 12, 8), 3
+    g = fields.gauge_field(L, N, seed=141)
+    psi = fields.gaussian_spinors(L, N, k, seed=142 + k)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+    tol = {_lib.HALF: 3e-3, _lib.BF16: 2e-2, _lib.DOUBLE: TOL[2]}[prec]
+    n = 0
+    for grid in ((1, 1, 1, 1), (1, 1, 2, 2)):
+        for d in range(4):
+            geom.grid[d] = grid[d]
+        for i, t in enumerate(col_tiles(L, k, Rs=(R,), blocks=(128,))):
+            parity, dagger = ((0, 1), (1, 0))[i & 1]
+            pout = torch.zeros_like(pin)
+            _lib.check(lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger,
+                                         geom, _lib.tile(*t), stream()))
+            got = unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+            assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger, grid=grid)) < tol, (grid, t)
+            n += 1
+    assert n >= 8
+
+
+def test_column_order_norms_and_fallbacks(lib, dev):
+    """A column-order tile set for a shape serves qcuMrhsDslashEx (per-RHS
+    norms from per-workgroup partials indexed by the logical workgroup); on
+    calls no hypercube kernel serves (two-row and eight-real links, N = 5) it
+    runs the default traversal; both against the oracle."""
+    from test_gauge12_gpu import run12
+    from test_gauge8_gpu import run8
+
+    L, N, k = (8, 8, 12, 8), 3, 16
+    g = fields.gauge_field(L, N, seed=151)
+    psi = fields.gaussian_spinors(L, N, k, seed=152)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, _lib.SINGLE)
+    t = _lib.tile(2, 128, 1, 2, 3, 0, 0, HC | COL)
+    _lib.check(lib.qcuMrhsSetTile(geom, _lib.SINGLE, k, ctypes.byref(t)))
+    pout = torch.zeros_like(pin)
+    nrm = torch.zeros(k, dtype=torch.float64, device=dev)
+    b = -0.25
+    for parity, dagger in ((0, 1), (1, 0)):
+        _lib.check(lib.qcuMrhsDslashEx(pout.data_ptr(), _lib.SINGLE, pin.data_ptr(), None, _lib.SINGLE, 0.0, b,
+                                       None, None, None, nrm.data_ptr(), gnat.data_ptr(), k, parity, dagger, geom,
+                                       stream()))
+        got = unpack(lib, dev, pout, _lib.SINGLE, k, geom, psi.shape[1:])
+        want = b * oracle.dslash(g, psi, L, parity, dagger)
+        assert rel_err(got, want) < TOL[1], (parity, dagger)
+        wn = np.array([np.vdot(want[r], want[r]).real for r in range(k)])
+        assert np.max(np.abs(nrm.cpu().numpy() - wn) / wn) < 4e-6, (parity, dagger)
+    for run in (run12, run8):
+        got = run(lib, dev, g, psi, L, 1, 0, _lib.SINGLE, -1, tile=t)
+        assert rel_err(got, oracle.dslash(g, psi, L, 1, 0)) < TOL[1], run.__name__
+    L5, N5, k5 = (4, 4, 4, 8), 5, 4
+    g5 = fields.gauge_field(L5, N5, seed=153)
+    psi5 = fields.gaussian_spinors(L5, N5, k5, seed=154)
+    geom5, gnat5, pin5 = pack_inputs(lib, dev, g5, psi5, L5, N5, _lib.SINGLE)
+    pout5 = torch.zeros_like(pin5)
+    _lib.check(lib.qcuMrhsDslash(pout5.data_ptr(), pin5.data_ptr(), gnat5.data_ptr(), _lib.SINGLE, k5, 0, 0, geom5,
+                                 _lib.tile(1, 128, 1, 1, 1, 0, 0, HC | COL), stream()))
+    got = unpack(lib, dev, pout5, _lib.SINGLE, k5, geom5, psi5.shape[1:])
+    assert rel_err(got, oracle.dslash(g5, psi5, L5, 0, 0)) < TOL[1]

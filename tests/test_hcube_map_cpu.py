@@ -17,6 +17,15 @@ import pytest
 
 def decode(hb, A):
     subLx, bLy, zb, tb, nzb = A["subLx"], A["bLy"], A["zblock"], A["tblock"], A["nzb"]
+    if A.get("col"):
+        # column order: ((column * nT + T) * slab + (z_in * cy + y_in) * subLx + xh)
+        cy, cz = A["col"]
+        q, b = divmod(hb, subLx * cy * cz)
+        col, T = divmod(q, A["Lt"] // 2)
+        r, xh = divmod(b, subLx)
+        zin, yin = divmod(r, cy)
+        cz_i, cy_i = divmod(col, bLy // cy)
+        return xh, cy_i * cy + yin, cz_i * cz + zin, T
     if A["ystream"]:
         q, xh = divmod(hb, subLx)
         q2, zin = divmod(q, zb)
@@ -54,11 +63,22 @@ def canonical(idx, A, p):
     return nb, live
 
 
-def check_launch(L, P, B, tb, zb, ystream, p, skip):
+def col_remap(hw, A, B, P):
+    """Logical workgroup of hardware workgroup hw in column order (None: idle)."""
+    cy, cz = A["col"]
+    ncol = (A["bLy"] // cy) * (A["bLz"] // cz)
+    grp = (A["Lt"] // 2) * (A["subLx"] * cy * cz // (B // (8 * P)))
+    i, x = hw >> 3, hw & 7
+    g = i // grp
+    col = 8 * g + x
+    return col * grp + (i - g * grp) if col < ncol else None
+
+
+def check_launch(L, P, B, tb, zb, ystream, p, skip, col=None):
     subLx, Ly, Lz, Lt = L[0] // 2, L[1], L[2], L[3]
     S = subLx * Ly * Lz * Lt
     A = dict(subLx=subLx, Ly=Ly, Lz=Lz, Lt=Lt, bLy=Ly // 2, bLz=Lz // 2, zblock=zb or Lz // 2, tblock=tb,
-             nzb=(Lz // 2) // (zb or Lz // 2), ystream=ystream, skip=skip)
+             nzb=(Lz // 2) // (zb or Lz // 2), ystream=ystream, skip=skip, col=col)
     lp = P.bit_length() - 1
     lpl, gb = min(lp, 3), 3 + lp
     nlanes = S * P
@@ -113,6 +133,33 @@ def check_launch(L, P, B, tb, zb, ystream, p, skip):
         want = {(D >> 1, 1 if D % 2 == 0 else -1, nb[D], (D >> 1) if D % 2 == 0 else 4 + (D >> 1), lv[D])
                 for D in range(8)}
         assert terms == want, (L, P, B, tb, zb, ystream, p, gid)
+    if col:
+        # The padded hardware grid maps onto every logical workgroup once; XCD
+        # hw % 8 works on columns congruent to it, one slab after another with T
+        # ascending, and the blocks of a slab stay inside its column.
+        cy, cz = col
+        ncol = (Ly // 2 // cy) * (Lz // 2 // cz)
+        nb = nlanes // B
+        grp = (Lt // 2) * (subLx * cy * cz // (B // (8 * P)))
+        assert nlanes % B == 0 and nb == ncol * grp
+        ngrid = (ncol + 7) // 8 * 8 * grp
+        got = [col_remap(hw, A, B, P) for hw in range(ngrid)]
+        live = [g for g in got if g is not None]
+        assert sorted(live) == list(range(nb))
+        slab_wg = grp // (Lt // 2)
+        for x in range(8):
+            seq = [g for hw, g in enumerate(got) if hw % 8 == x and g is not None]
+            cols = [g // grp for g in seq]
+            assert all(c % 8 == x for c in cols)
+            for a_, b_ in zip(seq, seq[1:]):
+                if a_ // grp == b_ // grp:
+                    assert (b_ % grp) // slab_wg >= (a_ % grp) // slab_wg
+        for blk in range(nb):
+            c = blk // grp
+            cyi, czi = c % (Ly // 2 // cy), c // (Ly // 2 // cy)
+            for gid in range(blk * B, blk * B + B):
+                Y, Z = lanes[gid][4] // 2, lanes[gid][5] // 2
+                assert cyi * cy <= Y < (cyi + 1) * cy and czi * cz <= Z < (czi + 1) * cz
 
 
 @pytest.mark.parametrize("L", [(4, 4, 4, 4), (2, 2, 2, 2), (8, 4, 6, 2), (6, 2, 4, 8)])
@@ -123,5 +170,21 @@ def test_hypercube_lane_map(L):
         for tb, zb in itertools.product([d for d in (1, 2, 3, 4) if bt % d == 0], [0] + [d for d in (1, 3) if bz % d == 0]):
             for ystream, p, skip in ((0, 0, (0, 0, 0, 0)), (1, 1, (1, 1, 1, 1)), (1, 0, (1, 0, 1, 0))):
                 check_launch(L, P, B, tb, zb, ystream, p, skip)
+                n += 1
+    assert n > 0
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (8, 4, 6, 2), (6, 8, 4, 4), (4, 12, 12, 4)])
+def test_column_order_lane_map(L):
+    """Column order (traversal 6): the same lane-level checks, plus the
+    hardware-to-logical workgroup map of the padded grid."""
+    n = 0
+    for P, B in ((1, 64), (2, 128), (8, 128), (8, 256), (16, 128)):
+        by, bz = L[1] // 2, L[2] // 2
+        for cy, cz in itertools.product([d for d in (1, 2, 3) if by % d == 0], [d for d in (1, 2, 3) if bz % d == 0]):
+            if (L[0] // 2 * cy * cz) % (B // (8 * P)):
+                continue
+            for p, skip in ((0, (0, 0, 0, 0)), (1, (1, 0, 1, 1))):
+                check_launch(L, P, B, 1, 0, 0, p, skip, col=(cy, cz))
                 n += 1
     assert n > 0
