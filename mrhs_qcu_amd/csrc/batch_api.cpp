@@ -125,12 +125,12 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       t.occ = 0;
       t.hcube = 0;  // the work-list body is the site-major kernel (the blocking still divides the lattice)
       if (t.trav_hi) {
-        // A column-order tile's blocking is in (y, z) blocks: default traversal instead.
+        // A column-order tile: y-streamed blocks of one t slice and z_block
+        // z slices instead (as launch_dslash runs it without a hypercube kernel).
         t.trav_hi = 0;
-        t.ystream = 0;
+        t.ystream = 1;
         t.tblock = 1;
-        t.zblock = 0;
-        t.hints = 0;
+        t.hints = 8;
       }
     }
   }

@@ -375,18 +375,20 @@ static int check_call(const DslashCall &c, std::string *err)
 
 // A column-order tile (traversal bit 2) on a call that no hypercube kernel
 // serves (compressed links, pointer layouts, precisions / N / rhs_per_lane
-// without one) runs with the default traversal of its rhs_per_lane, block
-// and link staging instead.
+// without one) runs the site-major kernel of its rhs_per_lane, block and
+// link staging, y-streamed over blocks of one t slice and z_block z slices
+// in the XCD-cyclic order where that divides the launch (at 24^3x48, 16 RHS,
+// FP32 two-row links with the 3 x 3 column tile: 324 us, against 390 us in
+// the natural order).
 static DslashCall effective_call(const DslashCall &c0)
 {
   if (!c0.tile.trav_hi || runs_hcube(c0)) return c0;
   DslashCall c = c0;
   c.tile.trav_hi = 0;
   c.tile.hcube = 0;
-  c.tile.ystream = 0;
+  c.tile.ystream = 1;
   c.tile.tblock = 1;
-  c.tile.zblock = 0;
-  c.tile.hints = 0;
+  c.tile.hints = 8;
   return c;
 }
 
