@@ -123,15 +123,10 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       t.lds = lds;
       t.block = kBatchBlock;
       t.occ = 0;
-      t.hcube = 0;  // the work-list body is the site-major kernel (the blocking still divides the lattice)
-      if (t.trav_hi) {
-        // A column-order tile: y-streamed blocks of one t slice and z_block
-        // z slices instead (as launch_dslash runs it without a hypercube kernel).
-        t.trav_hi = 0;
-        t.ystream = 1;
-        t.tblock = 1;
-        t.hints = 8;
-      }
+      // The work-list body is the site-major kernel: a column-order
+      // hypercube tile runs its columns in sites (else y-streamed blocks);
+      // other blockings still divide the lattice.
+      t = site_major_tile(t, geoms[i], b->probs[i].nrhs, SL_PACKED);
     }
   }
   if (!worklist)
@@ -178,7 +173,7 @@ int qcuMrhsBatchCreate(const qcuMrhsProblem *probs, int nprob, void **handle, vo
       }
       it.first = total;
       it.n = geoms[i].N;
-      total += (it.a.nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+      total += (it.a.ngrid + kNumXcd - 1) / kNumXcd * kNumXcd;
       shm_max = std::max(shm_max, shm);
       items.push_back(it);
     }

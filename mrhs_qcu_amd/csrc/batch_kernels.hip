@@ -11,8 +11,9 @@ __global__ void __launch_bounds__(256) batch_dslash_kernel(const BatchItem *__re
   while (p + 1 < nitems && hw >= items[p + 1].first) ++p;
   const DslashArgs a = items[p].a;
   const uint32_t local = hw - items[p].first;
-  // Padding workgroups of a range leave before any barrier.
-  if (local >= a.nblocks) return;
+  // Padding workgroups of a range leave before any barrier (the body's own
+  // column-order padding likewise).
+  if (local >= a.ngrid) return;
   switch (items[p].n) {
     case 2: dslash_body<2, 2, SL_PACKED, GL_NATIVE, true, false, c32, c32, c32, float>(a, local); break;
     case 3: dslash_body<3, 2, SL_PACKED, GL_NATIVE, true, false, c32, c32, c32, float>(a, local); break;

@@ -251,32 +251,6 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
     a.hdiv_row = make_fastdiv(uint32_t(a.subLx * a.bLy));
     a.hdiv_nzb = make_fastdiv(uint32_t(bz / a.zblock));
     a.hc_lp = uint32_t(__builtin_ctz(unsigned(P)));
-    if (c.tile.trav_hi == 1) {
-      // Column order: t_block / z_block are the column's extent in y / z
-      // blocks; a column's slab at one T is a whole number of workgroups.
-      const int cy = c.tile.tblock, cz = c.tile.zblock;
-      const int slab = a.subLx * cy * cz;
-      const int wpb = B / (8 * P);  // blocks per workgroup
-      if (slab % wpb) return fail(err, "column order: a column's slab must fill whole workgroups"), 1;
-      const uint32_t ncol = uint32_t((a.bLy / cy) * (bz / cz));
-      const uint32_t nT = uint32_t(c.geom.L[3] / 2);
-      const uint32_t ws = uint32_t(slab / wpb);
-      a.hc_col = 1;
-      a.col_cy = uint32_t(cy);
-      a.col_cz = uint32_t(cz);
-      a.col_ncol = ncol;
-      a.col_ncy = uint32_t(a.bLy / cy);
-      a.col_nT = nT;
-      a.col_slab = uint32_t(slab);
-      a.col_grp = nT * ws;
-      a.cdiv_slab = make_fastdiv(a.col_slab);
-      a.cdiv_nT = make_fastdiv(nT);
-      a.cdiv_cy = make_fastdiv(a.col_cy);
-      a.cdiv_ncy = make_fastdiv(a.col_ncy);
-      a.cdiv_grp = make_fastdiv(a.col_grp);
-    }
-  } else if (c.tile.trav_hi) {
-    return fail(err, "column order needs a hypercube kernel for these precisions / N / rhs_per_lane"), 1;
   }
   KernelKey key{SL_PACKED, c.gl, c.geom.N, R, lds, 0, c.prec_s, c.prec_g, c.prec_o};
   key.hc = hc;
@@ -297,15 +271,36 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   a.ngrid = a.nblocks;
   Tile tl = c.tile;
   tl.hcube = hc;
-  a.xcd_gran = a.hc_col ? 0u : xcd_cycle_gran(c.geom, tl, c.k);
-  if (a.hc_col) {
-    // Every logical workgroup belongs to one column slab (lanes divide into
-    // whole workgroups), and the grid pads the last group of 8 columns.
-    if (lanes % uint64_t(B) || uint64_t(a.nblocks) != uint64_t(a.col_ncol) * a.col_grp)
+  if (c.tile.trav_hi == 1) {
+    // Column order: t_block / z_block are a column's extent in y / z (blocks
+    // for the hypercube kernel, sites for the site-major one); every column
+    // is a whole number of workgroups.
+    const int cy = c.tile.tblock, cz = c.tile.zblock;
+    const int ey = hc ? c.geom.L[1] / 2 : c.geom.L[1], ez = hc ? c.geom.L[2] / 2 : c.geom.L[2];
+    const uint64_t nT = uint64_t(hc ? c.geom.L[3] / 2 : c.geom.L[3]);
+    const uint64_t slab = uint64_t(a.subLx) * cy * cz;
+    const uint64_t col_lanes = slab * nT * (hc ? 8ull * a.P : uint64_t(a.P));
+    if (col_lanes % uint64_t(B)) return fail(err, "column order: a column must fill whole workgroups"), 1;
+    a.col_order = 1;
+    a.col_cy = uint32_t(cy);
+    a.col_cz = uint32_t(cz);
+    a.col_ncol = uint32_t((ey / cy) * (ez / cz));
+    a.col_ncy = uint32_t(ey / cy);
+    a.col_nT = uint32_t(nT);
+    a.col_slab = uint32_t(slab);
+    a.col_grp = uint32_t(col_lanes / uint64_t(B));
+    a.cdiv_slab = make_fastdiv(a.col_slab);
+    a.cdiv_nT = make_fastdiv(a.col_nT);
+    a.cdiv_cy = make_fastdiv(a.col_cy);
+    a.cdiv_ncy = make_fastdiv(a.col_ncy);
+    a.cdiv_grp = make_fastdiv(a.col_grp);
+    if (uint64_t(a.nblocks) != uint64_t(a.col_ncol) * a.col_grp)
       return fail(err, "column order: the columns do not tile the lattice"), 1;
+    // The grid pads the last group of 8 columns with idle workgroups.
     a.ngrid = (a.col_ncol + 7u) / 8u * 8u * a.col_grp;
     a.xcd_map = 0;
   }
+  a.xcd_gran = a.col_order ? 0u : xcd_cycle_gran(c.geom, tl, c.k);
   size_t shm = 0;
   if (lds) {
     const size_t nsite_max = size_t((B - 1) / a.P + 2);
@@ -349,14 +344,17 @@ static int check_call(const DslashCall &c, std::string *err)
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
   if (c.tile.trav_hi > 1) return fail(err, "traversal bits 3 and above are not defined"), 1;
   if (c.tile.trav_hi) {
-    // Column order (hypercube, packed layout): t_block x z_block blocks in (y, z).
-    if (!c.tile.hcube || c.tile.ystream || c.sl != SL_PACKED)
-      return fail(err, "column order (traversal bit 2) is a hypercube traversal of the packed layout "
-                       "(traversal 6)"), 1;
-    if (c.tile.tblock < 1 || (c.geom.L[1] / 2) % c.tile.tblock)
-      return fail(err, "column order: t_block (column extent in y blocks) must divide Ly / 2"), 1;
-    if (c.tile.zblock < 1 || (c.geom.L[2] / 2) % c.tile.zblock)
-      return fail(err, "column order: z_block (column extent in z blocks) must divide Lz / 2"), 1;
+    // Column order of the packed layout: t_block x z_block blocks (traversal
+    // 6) or sites (traversal 4) in (y, z).
+    if (c.tile.ystream || c.sl != SL_PACKED)
+      return fail(err, "column order (traversal bit 2) is a traversal of the packed layout (traversal 4 or 6)"), 1;
+    const int f = c.tile.hcube ? 2 : 1;
+    if (c.tile.tblock < 1 || (c.geom.L[1] / f) % c.tile.tblock)
+      return fail(err, c.tile.hcube ? "column order: t_block (column extent in y blocks) must divide Ly / 2"
+                                    : "column order: t_block (column extent in y) must divide Ly"), 1;
+    if (c.tile.zblock < 1 || (c.geom.L[2] / f) % c.tile.zblock)
+      return fail(err, c.tile.hcube ? "column order: z_block (column extent in z blocks) must divide Lz / 2"
+                                    : "column order: z_block (column extent in z) must divide Lz"), 1;
   } else if (!c.tile.hcube) {
     if (c.tile.tblock > 1 && c.geom.L[3] % c.tile.tblock) return fail(err, "t_block must divide Lt"), 1;
     if (c.tile.zblock > 0 && c.geom.L[2] % c.tile.zblock) return fail(err, "z_block must divide Lz"), 1;
@@ -373,22 +371,35 @@ static int check_call(const DslashCall &c, std::string *err)
   return 0;
 }
 
-// A column-order tile (traversal bit 2) on a call that no hypercube kernel
-// serves (compressed links, pointer layouts, precisions / N / rhs_per_lane
-// without one) runs the site-major kernel of its rhs_per_lane, block and
-// link staging, y-streamed over blocks of one t slice and z_block z slices
-// in the XCD-cyclic order where that divides the launch (at 24^3x48, 16 RHS,
-// FP32 two-row links with the 3 x 3 column tile: 324 us, against 390 us in
-// the natural order).
+Tile site_major_tile(const Tile &t0, const Geom &g, int k, int sl)
+{
+  Tile t = t0;
+  t.hcube = 0;
+  if (!t0.trav_hi) return t;
+  const int R = t.R > 0 ? t.R : 1, B = t.block > 0 ? t.block : 256;
+  const int f = t0.hcube ? 2 : 1;  // column extents in sites
+  const long long col_lanes = 1LL * (g.L[0] / 2) * (f * t0.tblock) * (f * t0.zblock) * g.L[3] * (k / R);
+  if (sl == SL_PACKED && k % R == 0 && col_lanes % B == 0) {
+    // The same columns in sites (traversal 4).
+    t.tblock = f * t0.tblock;
+    t.zblock = f * t0.zblock;
+    return t;
+  }
+  t.trav_hi = 0;
+  t.ystream = 1;
+  t.tblock = 1;
+  t.hints = 8;
+  return t;
+}
+
+// A column-order hypercube tile (traversal 6) on a call that no hypercube
+// kernel serves (compressed links, precisions / N / rhs_per_lane without
+// one) runs the site-major kernel in the same columns (site_major_tile).
 static DslashCall effective_call(const DslashCall &c0)
 {
   if (!c0.tile.trav_hi || runs_hcube(c0)) return c0;
   DslashCall c = c0;
-  c.tile.trav_hi = 0;
-  c.tile.hcube = 0;
-  c.tile.ystream = 1;
-  c.tile.tblock = 1;
-  c.tile.hints = 8;
+  c.tile = site_major_tile(c0.tile, c0.geom, c0.k, c0.sl);
   return c;
 }
 

@@ -108,9 +108,16 @@ struct Tile {
   int ystream = 0; // 1: stream each (tblock, zblock) block along y
   int hcube = 0;   // 1: hypercube work mapping (2x2x2x2 blocks, one per 8 P lanes; tblock / zblock
                    // then count blocks)
-  int trav_hi = 0; // traversal bits 2 and above: 1 = column order of the hypercube mapping (tblock /
-                   // zblock = a column's extent in y / z blocks; XCD x walks columns 8 g + x along t)
+  int trav_hi = 0; // traversal bits 2 and above: 1 = column order (tblock / zblock = a column's extent
+                   // in y / z, in sites, or in blocks with hcube; XCD x walks columns 8 g + x along t)
 };
+
+// The tile a site-major kernel runs for tile t (hcube cleared): a
+// column-order tile (traversal 4, or 6 with its extents in blocks doubled to
+// sites) keeps its columns when they fill whole workgroups of the packed
+// layout, else it becomes y-streamed blocks of one t slice and z_block z
+// slices in the XCD-cyclic order.
+Tile site_major_tile(const Tile &t, const Geom &g, int k, int sl);
 
 // The traversal field of qcuMrhsTile: bit 0 ystream, bit 1 hcube, bit 2 column order.
 inline int tile_traversal(const Tile &t) { return t.ystream | (t.hcube << 1) | (t.trav_hi << 2); }

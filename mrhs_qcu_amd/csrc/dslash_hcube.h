@@ -39,7 +39,7 @@ namespace qmrhs {
 __device__ __forceinline__ void hc_decode(uint32_t hb, const DslashArgs &a, uint32_t &xh, uint32_t &Y, uint32_t &Z,
                                           uint32_t &T)
 {
-  if (a.hc_col) {
+  if (a.col_order) {
     const uint32_t q = fdiv(hb, a.cdiv_slab);  // column * nT + T
     const uint32_t b = hb - q * a.col_slab;
     const uint32_t col = fdiv(q, a.cdiv_nT);
@@ -82,17 +82,6 @@ __device__ __forceinline__ void hc_decode(uint32_t hb, const DslashArgs &a, uint
   const uint32_t zout = q3 - tout * nzb;
   Z = zout * a.zblock + zin;
   T = tout * a.tblock + tin;
-}
-
-// Column order: the logical workgroup of hardware workgroup hw (XCD hw % 8
-// walks column 8 g + hw % 8, g = (hw / 8) / col_grp), or 0xffffffff for a
-// workgroup past the last column.
-__device__ __forceinline__ uint32_t col_remap(uint32_t hw, const DslashArgs &a)
-{
-  const uint32_t i = hw >> 3;
-  const uint32_t g = fdiv(i, a.cdiv_grp);
-  const uint32_t col = 8u * g + (hw & 7u);
-  return col < a.col_ncol ? col * a.col_grp + (i - g * a.col_grp) : 0xffffffffu;
 }
 
 struct HcSite {
@@ -301,7 +290,7 @@ __device__ __forceinline__ void dslash_hc_body(const DslashArgs &a, uint32_t hw_
 {
   static_assert(N < kRowLinkN, "hypercube kernels hold whole links in registers");
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t blk = a.hc_col ? col_remap(hw_blk, a)
+  const uint32_t blk = a.col_order ? col_remap(hw_blk, a)
                                 : (a.xcd_gran ? xcd_cyclic(hw_blk, a.xcd_gran)
                                               : (a.xcd_map ? xcd_remap(hw_blk, a.nblocks) : hw_blk));
   // Idle column-order workgroups (past the last column) leave before any

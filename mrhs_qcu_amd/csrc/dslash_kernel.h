@@ -103,14 +103,17 @@ struct DslashArgs {
   uint32_t hc_lp;
   int bLy, bLz;
   FastDiv hdiv_Ly, hdiv_row, hdiv_nzb;
-  // Column order of the hypercube kernels (traversal bit 2): the block
-  // lattice is cut into columns of col_cy x col_cz blocks in (y, z), whole in
-  // x and t; the logical block number is ((column * nT + T) * slab + b) with
-  // b = (z_in * col_cy + y_in) * subLx + xh inside the column's slab at T.
-  // XCD x walks column 8 g + x slab by slab from T = 0 to nT - 1, for
-  // g = 0, 1, ... (col_remap); the launch has 8 ceil(col_ncol / 8) col_grp
-  // workgroups, the ones past the last column idle.
-  int hc_col;
+  // Column order (traversal bit 2): the lattice (site-major kernels, units
+  // of sites, nT = Lt) or the block lattice (hypercube kernels, units of
+  // 2x2x2x2 blocks, nT = Lt / 2) is cut into columns of col_cy x col_cz in
+  // (y, z), whole in x and t; the logical site (block) number is
+  // ((column * nT + T) * slab + b) with b = (z_in * col_cy + y_in) * subLx +
+  // xh inside the column's slab at time T. Every column is a whole number
+  // (col_grp) of workgroups; XCD x walks column 8 g + x slab by slab from
+  // T = 0 to nT - 1, for g = 0, 1, ... (col_remap); the launch has
+  // 8 ceil(col_ncol / 8) col_grp workgroups, the ones past the last column
+  // idle.
+  int col_order;
   uint32_t col_cy, col_cz, col_ncol, col_ncy, col_nT, col_slab, col_grp;
   FastDiv cdiv_slab, cdiv_nT, cdiv_cy, cdiv_ncy, cdiv_grp;
   uint32_t ngrid;  // workgroups launched: nblocks, or the padded column-order grid
@@ -284,6 +287,17 @@ struct SiteGeo {
 __device__ __forceinline__ float lane_read(float v, int src) { return __shfl(v, src, 64); }
 __device__ __forceinline__ double lane_read(double v, int src) { return __shfl(v, src, 64); }
 
+// Column order: the logical workgroup of hardware workgroup hw (XCD hw % 8
+// walks column 8 g + hw % 8, g = (hw / 8) / col_grp), or 0xffffffff for a
+// workgroup past the last column.
+__device__ __forceinline__ uint32_t col_remap(uint32_t hw, const DslashArgs &a)
+{
+  const uint32_t i = hw >> 3;
+  const uint32_t g = fdiv(i, a.cdiv_grp);
+  const uint32_t col = 8u * g + (hw & 7u);
+  return col < a.col_ncol ? col * a.col_grp + (i - g * a.col_grp) : 0xffffffffu;
+}
+
 // Logical site number -> coordinates for the traversal orders
 //   (t_out, z_out, t_in, z_in, y, x/2)   blocks of t_block x z_block slices, or
 //   (t_out, z_out, y, t_in, z_in, x/2)   the same blocks streamed along y, so
@@ -292,6 +306,20 @@ __device__ __forceinline__ double lane_read(double v, int src) { return __shfl(v
 __device__ __forceinline__ void decode_site(uint32_t sl, const DslashArgs &a, uint32_t &xh, uint32_t &y, uint32_t &z,
                                             uint32_t &t)
 {
+  if (a.col_order) {
+    const uint32_t q = fdiv(sl, a.cdiv_slab);  // column * Lt + t
+    const uint32_t b = sl - q * a.col_slab;
+    const uint32_t col = fdiv(q, a.cdiv_nT);
+    t = q - col * a.col_nT;
+    const uint32_t r = fdiv(b, a.div_subLx);
+    xh = b - r * a.subLx;
+    const uint32_t zin = fdiv(r, a.cdiv_cy);
+    const uint32_t yin = r - zin * a.col_cy;
+    const uint32_t cz = fdiv(col, a.cdiv_ncy);
+    y = (col - cz * a.col_ncy) * a.col_cy + yin;
+    z = cz * a.col_cz + zin;
+    return;
+  }
   if (a.ystream) {
     const uint32_t q = fdiv(sl, a.div_subLx);
     xh = sl - q * a.subLx;
@@ -1235,7 +1263,12 @@ template <int N, int R, int SL, int GL, bool LDS, bool SITE_MINOR, class CS, cla
 __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk)
 {
   extern __shared__ __align__(16) unsigned char smem[];
-  const uint32_t blk = a.xcd_gran ? xcd_cyclic(hw_blk, a.xcd_gran) : (a.xcd_map ? xcd_remap(hw_blk, a.nblocks) : hw_blk);
+  const uint32_t blk = a.col_order ? col_remap(hw_blk, a)
+                                   : (a.xcd_gran ? xcd_cyclic(hw_blk, a.xcd_gran)
+                                                 : (a.xcd_map ? xcd_remap(hw_blk, a.nblocks) : hw_blk));
+  // Idle column-order workgroups (past the last column) leave before any
+  // barrier; the whole workgroup takes this branch.
+  if (blk >= a.nblocks) return;
   const uint32_t lane0 = blk * blockDim.x;
   const uint32_t gid = lane0 + threadIdx.x;
   const bool active = gid < a.nlanes;

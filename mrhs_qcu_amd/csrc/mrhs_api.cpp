@@ -131,7 +131,7 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.ystream = v[15] & 1;
     t.hcube = (v[15] >> 1) & 1;
     t.trav_hi = v[15] >> 2;
-    if (t.trav_hi > 1 || (t.trav_hi && (!t.hcube || t.ystream))) continue;  // not a defined traversal
+    if (t.trav_hi > 1 || (t.trav_hi && t.ystream)) continue;  // not a defined traversal
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -269,6 +269,23 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
             // The same tile with the XCDs side by side on adjacent blocks.
             t.hints = 8;
             if (xcd_cycle_gran(g, t, k)) out.push_back(t);
+          }
+        }
+        // Column order in sites (traversal 4): columns of cy x cz sites in
+        // (y, z), XCD x walking columns 8 g + x along t; only column counts
+        // that fill every XCD, each column whole workgroups.
+        for (int cy : {2, 3, 4, 6, 8, 12}) {
+          for (int cz : {2, 3, 4, 6, 8, 12}) {
+            if (g.L[1] % cy || g.L[2] % cz || ((g.L[1] / cy) * (g.L[2] / cz)) % 8) continue;
+            if ((1LL * (g.L[0] / 2) * cy * cz * g.L[3] * (k / R)) % block) continue;
+            Tile t;
+            t.R = R;
+            t.lds = lds;
+            t.block = block;
+            t.tblock = cy;
+            t.zblock = cz;
+            t.trav_hi = 1;
+            out.push_back(t);
           }
         }
       }
@@ -641,7 +658,7 @@ int qcuMrhsSetTile(const qcuMrhsGeom *gc, int prec, int nrhs, const qcuMrhsTile 
   if (t.R != 1 && t.R != 2 && t.R != 4) return api_fail("rhs_per_lane must be 1, 2 or 4");
   if (t.block < 64 || t.block > 256 || t.block % 64) return api_fail("block must be 64, 128, 192 or 256");
   if (t.trav_hi > 1) return api_fail("traversal bits 3 and above are not defined");
-  if (t.trav_hi && (!t.hcube || t.ystream)) return api_fail("column order (traversal bit 2) needs traversal 6");
+  if (t.trav_hi && t.ystream) return api_fail("column order (traversal bit 2) is traversal 4 or 6");
   set_tile(g, prec, nrhs, t);
   return 0;
 }

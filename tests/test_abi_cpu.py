@@ -288,7 +288,7 @@ def test_pinned_tile_table():
         assert len(f) == 17 and f[0] == "gfx950", line
         N, k, prec, Lx, Ly, Lz, Lt, mask, R, block, lds, tb, zb, hints, occ, trav = map(int, f[1:])
         assert k % R == 0 and block in (64, 128, 192, 256) and lds in (0, 1), line
-        assert trav in (0, 1, 2, 3, 6), line
+        assert trav in (0, 1, 2, 3, 4, 6), line
         if trav & 2:
             # hypercube mapping: blocking counted in 2x2x2x2 blocks, whole
             # blocks of 8 P lanes per workgroup, links staged in LDS
@@ -300,6 +300,10 @@ def test_pinned_tile_table():
             by, bz = Ly // 2, Lz // 2
             assert by % tb == 0 and bz % zb == 0 and (by // tb) * (bz // zb) % 8 == 0, line
             assert (Lx // 2 * tb * zb) % (block // (8 * (k // R))) == 0, line
+        elif trav == 4:
+            # column order in sites
+            assert Ly % tb == 0 and Lz % zb == 0 and (Ly // tb) * (Lz // zb) % 8 == 0, line
+            assert (Lx // 2 * tb * zb * Lt * (k // R)) % block == 0, line
         else:
             assert Lt % max(tb, 1) == 0 and (zb == 0 or Lz % zb == 0), line
             if trav & 2:
@@ -327,7 +331,7 @@ def test_set_tile_round_trip(lib):
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 96))) != 0
     assert b"block" in lib.qcuMrhsLastError()
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 128, 1, 1, 1, 8, 0, 3 | 4))) != 0
-    assert b"traversal 6" in lib.qcuMrhsLastError()
+    assert b"traversal 4 or 6" in lib.qcuMrhsLastError()
     assert lib.qcuMrhsSetTile(g, _lib.SINGLE, 8, ctypes.byref(_lib.tile(2, 128, 1, 1, 1, 8, 0, 2 | 8))) != 0
     assert b"traversal bits 3" in lib.qcuMrhsLastError()
     col = (2, 128, 1, 2, 5, 0, 0, 6)  # column order: 2 x 5 blocks in (y, z)

@@ -23,9 +23,23 @@ PROBLEMS = [((8, 8, 8, 8), 2, 8, 0, 0), ((8, 8, 8, 16), 3, 12, 1, 0), ((8, 4, 4,
             ((4, 4, 4, 8), 5, 6, 1, 1)]
 
 
-@pytest.mark.parametrize("problems", [PROBLEMS, [((4, 4, 4, 8), 6, 4, 1, 0), ((8, 4, 4, 8), 3, 10, 0, 1)]],
-                         ids=["worklist", "chain"])
+# The same colour counts on other shapes, with column-order tiles set for them
+# (traversal 4: columns of cy x cz sites in (y, z), XCD x walking columns
+# 8 g + x along t), in the work list's tile family (R = 2 with links in LDS
+# for N <= 4, R = 1 for N = 5; 128 lanes per workgroup).
+COLUMN_PROBLEMS = [((8, 8, 16, 8), 2, 8, 1, 0), ((8, 16, 8, 8), 3, 12, 0, 1), ((8, 8, 8, 8), 4, 24, 1, 1),
+                   ((4, 8, 8, 4), 5, 6, 0, 0)]
+COLUMN_TILES = [(2, 128, 1, 2, 4, 0, 0, 4), (2, 128, 1, 4, 2, 0, 0, 4), (2, 128, 1, 2, 2, 0, 0, 4),
+                (1, 128, 0, 4, 2, 0, 0, 4)]
+
+
+@pytest.mark.parametrize("problems", [PROBLEMS, [((4, 4, 4, 8), 6, 4, 1, 0), ((8, 4, 4, 8), 3, 10, 0, 1)],
+                                      COLUMN_PROBLEMS],
+                         ids=["worklist", "chain", "worklist_columns"])
 def test_batch_graph_replays(lib, dev, problems):
+    if problems is COLUMN_PROBLEMS:
+        for (L, N, k, _, _), t in zip(problems, COLUMN_TILES):
+            _lib.check(lib.qcuMrhsSetTile(_lib.geom(L, N), _lib.SINGLE, k, ctypes.byref(_lib.tile(*t))))
     items = []
     for i, (L, N, k, parity, dagger) in enumerate(problems):
         g = fields.gauge_field(L, N, seed=40 + i)
@@ -48,6 +62,8 @@ def test_batch_graph_replays(lib, dev, problems):
         for i, it in enumerate(items):
             t = _lib.Tile()  # the tile the batch runs this problem with
             _lib.check(lib.qcuMrhsBatchGetTile(handle, i, ctypes.byref(t)))
+            if problems is COLUMN_PROBLEMS:
+                assert t.as_tuple() == COLUMN_TILES[i], (i, t.as_tuple())
             o = torch.empty_like(it["pin"])
             _lib.check(lib.qcuMrhsDslash(o.data_ptr(), it["pin"].data_ptr(), it["gnat"].data_ptr(), _lib.SINGLE,
                                          it["k"], it["parity"], it["dagger"], it["geom"], ctypes.byref(t), stream()))

@@ -145,7 +145,7 @@ def test_hcube_rejects_bad_tiles(lib, dev):
                    ((2, 128, 1, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
                    ((2, 128, 0, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
                    ((2, 128, 1, 1, 0, 0, 0, HC | 8), "traversal bits 3 and above"),
-                   ((2, 128, 1, 1, 1, 0, 0, HC | COL | 1), "traversal 6"),
+                   ((2, 128, 1, 1, 1, 0, 0, HC | COL | 1), "traversal 4 or 6"),
                    ((2, 128, 1, 4, 1, 0, 0, HC | COL), "column order: t_block"),
                    ((2, 128, 1, 1, 2, 0, 0, HC | COL), "column order: z_block")):
         rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), _lib.SINGLE, k, 0, 0, geom,
@@ -462,3 +462,40 @@ def test_column_order_norms_and_fallbacks(lib, dev):
                                  _lib.tile(1, 128, 1, 1, 1, 0, 0, HC | COL), stream()))
     got = unpack(lib, dev, pout5, _lib.SINGLE, k5, geom5, psi5.shape[1:])
     assert rel_err(got, oracle.dslash(g5, psi5, L5, 0, 0)) < TOL[1]
+
+
+@pytest.mark.parametrize("N,L,k,R,prec", [(3, (8, 8, 12, 8), 16, 2, _lib.SINGLE), (3, (8, 8, 12, 8), 12, 2, _lib.SINGLE),
+                                          (2, (8, 4, 8, 8), 8, 1, _lib.SINGLE), (4, (4, 8, 4, 8), 6, 2, _lib.SINGLE),
+                                          (5, (4, 4, 8, 8), 4, 1, _lib.SINGLE), (3, (8, 8, 12, 8), 4, 1, _lib.DOUBLE),
+                                          (3, (8, 8, 12, 8), 16, 4, _lib.HALF)])
+def test_site_major_column_order(lib, dev, N, L, k, R, prec):
+    """Column order in sites (traversal 4, the site-major kernels): every
+    column shape whose columns fill whole workgroups, N = 2..5, k = 4..16,
+    FP32 / FP64 / FP16, a split lattice included, against the oracle;
+    shapes whose columns do not fill whole workgroups run y-streamed blocks."""
+    g = fields.gauge_field(L, N, seed=161 + N)
+    psi = fields.gaussian_spinors(L, N, k, seed=162 + k)
+    geom, gnat, pin = pack_inputs(lib, dev, g, psi, L, N, prec)
+    tol = {_lib.HALF: 3e-3, _lib.SINGLE: TOL[1], _lib.DOUBLE: TOL[2]}[prec]
+    n = 0
+    for grid in ((1, 1, 1, 1), (1, 2, 1, 2)):
+        for d in range(4):
+            geom.grid[d] = grid[d]
+        i = 0
+        for cy in [d for d in (1, 2, 4, 8) if L[1] % d == 0]:
+            for cz in [d for d in (1, 2, 3, 4, 12) if L[2] % d == 0]:
+                for block, lds in ((128, 1), (256, 0)):
+                    parity, dagger = ((0, 1), (1, 0))[i & 1]
+                    i += 1
+                    t = (R, block, lds, cy, cz, 0, 0, COL)
+                    pout = torch.full_like(pin, float("nan"))
+                    rc = lib.qcuMrhsDslash(pout.data_ptr(), pin.data_ptr(), gnat.data_ptr(), prec, k, parity, dagger,
+                                           geom, _lib.tile(*t), stream())
+                    if rc != 0:
+                        msg = lib.qcuMrhsLastError().decode()
+                        assert "LDS" in msg or "no kernel instantiated" in msg, (t, msg)
+                        continue
+                    got = unpack(lib, dev, pout, prec, k, geom, psi.shape[1:])
+                    assert rel_err(got, oracle.dslash(g, psi, L, parity, dagger, grid=grid)) < tol, (grid, t)
+                    n += 1
+    assert n >= 8

@@ -188,3 +188,51 @@ def test_column_order_lane_map(L):
                 check_launch(L, P, B, 1, 0, 0, p, skip, col=(cy, cz))
                 n += 1
     assert n > 0
+
+
+def test_site_major_column_order_map():
+    """Column order in sites (traversal 4, dslash_kernel.h decode_site and
+    col_remap): logical site ((column * Lt + t) * slab + (z_in * cy + y_in) *
+    subLx + xh); the padded grid of workgroups maps onto every logical
+    workgroup once, each XCD on columns congruent to it with t ascending, and
+    the sites of a workgroup inside its column."""
+    n = 0
+    for L, P, B in (((8, 4, 6, 8), 2, 128), ((4, 8, 12, 6), 8, 128), ((6, 6, 4, 4), 1, 64), ((4, 12, 12, 8), 4, 256)):
+        subLx, Ly, Lz, Lt = L[0] // 2, L[1], L[2], L[3]
+        S = subLx * Ly * Lz * Lt
+        for cy in [d for d in (1, 2, 3, 4, 6) if Ly % d == 0]:
+            for cz in [d for d in (1, 2, 3, 4, 6) if Lz % d == 0]:
+                col_lanes = subLx * cy * cz * Lt * P
+                if col_lanes % B:
+                    continue
+                ncy, ncol = Ly // cy, (Ly // cy) * (Lz // cz)
+                grp = col_lanes // B
+                nb = S * P // B
+                assert nb == ncol * grp
+                seen = set()
+                for sl in range(S):
+                    q, b = divmod(sl, subLx * cy * cz)
+                    col, t = divmod(q, Lt)
+                    r, xh = divmod(b, subLx)
+                    zin, yin = divmod(r, cy)
+                    czi, cyi = divmod(col, ncy)
+                    y, z = cyi * cy + yin, czi * cz + zin
+                    assert 0 <= y < Ly and 0 <= z < Lz and t < Lt
+                    seen.add(((t * Lz + z) * Ly + y) * subLx + xh)
+                assert len(seen) == S
+                ngrid = (ncol + 7) // 8 * 8 * grp
+                got = []
+                for hw in range(ngrid):
+                    i, x = hw >> 3, hw & 7
+                    g = i // grp
+                    c = 8 * g + x
+                    got.append(c * grp + (i - g * grp) if c < ncol else None)
+                assert sorted(v for v in got if v is not None) == list(range(nb))
+                for hw, v in enumerate(got):
+                    if v is not None:
+                        assert (v // grp) % 8 == hw % 8
+                        # every lane of the workgroup is a site of column v // grp
+                        for gid in (v * B, v * B + B - 1):
+                            assert (gid // P) // (subLx * cy * cz * Lt) == v // grp
+                n += 1
+    assert n > 10
