@@ -217,9 +217,13 @@ __device__ __forceinline__ void hc_apply_p(cpx<float> (&acc)[4][N], const typena
   constexpr HopEntry e = hop_entry(2 * MU);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    cpx<float> g0{0.f, 0.f}, g1{0.f, 0.f};
+    cpx<float> g0, g1;
+    g0.re = P::dot0(Vc[i][0], hs[0][0]);
+    g0.im = P::dot0(V[i][0], hs[0][0]);
+    g1.re = P::dot0(Vc[i][0], hs[1][0]);
+    g1.im = P::dot0(V[i][0], hs[1][0]);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
+    for (int j = 1; j < N; ++j) {
       g0.re = P::dot(Vc[i][j], hs[0][j], g0.re);
       g0.im = P::dot(V[i][j], hs[0][j], g0.im);
       g1.re = P::dot(Vc[i][j], hs[1][j], g1.re);
