@@ -224,7 +224,8 @@ class Qcu {
       CHECK_HIP(hipMalloc(&buf, bytes));
     }
     hipStream_t st = streams().stream;
-    CHECK_HIP(hipMemcpyAsync(buf, fermionIn, bytes, hipMemcpyDeviceToDevice, st));
+    std::string err;
+    if (qmrhs::copy_device(buf, fermionIn, bytes, st, &err)) fatal(err);
     CHECK_HIP(hipStreamSynchronize(st));
     stagingQueue_.push_back(buf);
     outQueue_.push_back(fermionOut);

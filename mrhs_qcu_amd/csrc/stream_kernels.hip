@@ -9,6 +9,7 @@
 #include <string>
 
 #include "api_internal.h"
+#include "storage_api.h"
 
 namespace qmrhs {
 namespace {
@@ -63,6 +64,13 @@ __global__ void __launch_bounds__(kBlock) stream_kernel(f4 *dst, const f4 *src, 
   }
 }
 
+constexpr unsigned kCopyGrid = 1024;
+
+__global__ void __launch_bounds__(kBlock) copy_kernel(f4 *__restrict__ dst, const f4 *__restrict__ src, size_t n16)
+{
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n16; i += size_t(gridDim.x) * kBlock) dst[i] = src[i];
+}
+
 int launch_stream(int mode, void *dst, const void *src, size_t n16, hipStream_t st)
 {
   const size_t per_block = size_t(kBlock) * kUnroll;
@@ -80,6 +88,24 @@ int launch_stream(int mode, void *dst, const void *src, size_t n16, hipStream_t 
 }
 
 }  // namespace
+
+int copy_device(void *dst, const void *src, size_t bytes, hipStream_t st, std::string *err)
+{
+  if (bytes == 0) return 0;
+  if (bytes % 16 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16) {
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return *err = "device copy failed", 1;
+    return 0;
+  }
+  const size_t n16 = bytes / 16;
+  const size_t need = (n16 + kBlock - 1) / kBlock;
+  const unsigned grid = unsigned(need < kCopyGrid ? need : kCopyGrid);
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kBlock), 0, st, static_cast<f4 *>(dst), static_cast<const f4 *>(src),
+                     n16);
+  if (hipGetLastError() != hipSuccess) return *err = "device copy launch failed", 1;
+  return 0;
+}
+
 }  // namespace qmrhs
 
 using namespace qmrhs;
