@@ -301,7 +301,7 @@ std::vector<Tile> tile_candidates(const Geom &g, int prec, int k)
     KernelKey key_kf = key;
     key_kf.kf = k;
     if (!find_kernel(key) && !find_kernel(key_kf)) continue;
-    for (int block : {128, 256}) {
+    for (int block : {64, 128, 256}) {
       if (block % (8 * P)) continue;
       const size_t shm = size_t(block / P) * 8 * g.N * g.N * prec_bytes(gauge_prec_for(prec));
       if (shm > size_t(kMaxDynLds)) continue;
