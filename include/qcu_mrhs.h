@@ -65,12 +65,14 @@ typedef struct qcuMrhsTile_s {
                        P = nrhs / rhs_per_lane in {1, 2, 4, ..., 32}, block a multiple
                        of 8 P, link_lds = 1, nColors <= 4,
                        and t_block / z_block then count blocks (divisors of Lt/2, Lz/2);
-                       bit 2 (with bit 1, bit 0 clear: traversal 6): column order of the
-                       hypercube mapping, the block lattice cut into columns of t_block x
-                       z_block blocks in (y, z) (divisors of Ly/2, Lz/2), XCD x walking
-                       columns 8 g + x block-time slab by block-time slab; on calls without
-                       a hypercube kernel it runs y-streamed site-major blocks of one t
-                       slice and z_block z slices instead;
+                       bit 2 (bit 0 clear): column order, the lattice cut into columns of
+                       t_block x z_block in (y, z) that XCD x walks slab by slab along t
+                       (columns 8 g + x), each column a whole number of workgroups;
+                       traversal 4: site-major kernels, columns in sites (divisors of Ly,
+                       Lz); traversal 6: the hypercube mapping, columns in blocks (divisors
+                       of Ly/2, Lz/2), run on calls without a hypercube kernel as the same
+                       columns in sites, or where those do not fill whole workgroups as
+                       y-streamed site-major blocks of one t slice and z_block z slices;
                        higher bits are not defined (a tile that sets them is rejected) */
 } qcuMrhsTile;
 
