@@ -73,6 +73,7 @@ typedef struct qcuMrhsTile_s {
                        of Ly/2, Lz/2), run on calls without a hypercube kernel as the same
                        columns in sites, or where those do not fill whole workgroups as
                        y-streamed site-major blocks of one t slice and z_block z slices;
+                       bit 3 (with bit 2 only): the columns also split x in two halves;
                        higher bits are not defined (a tile that sets them is rejected) */
 } qcuMrhsTile;
 

@@ -271,20 +271,26 @@ int prepare_packed(const DslashCall &c, DslashArgs *ap, size_t *shm_out, LaunchF
   a.ngrid = a.nblocks;
   Tile tl = c.tile;
   tl.hcube = hc;
-  if (c.tile.trav_hi == 1) {
+  if (c.tile.trav_hi) {
     // Column order: t_block / z_block are a column's extent in y / z (blocks
-    // for the hypercube kernel, sites for the site-major one); every column
-    // is a whole number of workgroups.
+    // for the hypercube kernel, sites for the site-major one), x whole or
+    // (traversal bit 3) halved; every column is a whole number of workgroups.
     const int cy = c.tile.tblock, cz = c.tile.zblock;
+    const int nx = (c.tile.trav_hi & 2) ? 2 : 1;
+    if (a.subLx % nx) return fail(err, "column order: splitting x needs Lx / 2 even"), 1;
     const int ey = hc ? c.geom.L[1] / 2 : c.geom.L[1], ez = hc ? c.geom.L[2] / 2 : c.geom.L[2];
     const uint64_t nT = uint64_t(hc ? c.geom.L[3] / 2 : c.geom.L[3]);
-    const uint64_t slab = uint64_t(a.subLx) * cy * cz;
+    const uint64_t slab = uint64_t(a.subLx / nx) * cy * cz;
     const uint64_t col_lanes = slab * nT * (hc ? 8ull * a.P : uint64_t(a.P));
     if (col_lanes % uint64_t(B)) return fail(err, "column order: a column must fill whole workgroups"), 1;
     a.col_order = 1;
     a.col_cy = uint32_t(cy);
     a.col_cz = uint32_t(cz);
-    a.col_ncol = uint32_t((ey / cy) * (ez / cz));
+    a.col_ncol = uint32_t((ey / cy) * (ez / cz) * nx);
+    a.col_sx = uint32_t(a.subLx / nx);
+    a.col_nx = uint32_t(nx);
+    a.cdiv_sx = make_fastdiv(a.col_sx);
+    a.cdiv_nx = make_fastdiv(a.col_nx);
     a.col_ncy = uint32_t(ey / cy);
     a.col_nT = uint32_t(nT);
     a.col_slab = uint32_t(slab);
@@ -342,7 +348,8 @@ static int check_call(const DslashCall &c, std::string *err)
                      "sides and tiles with rhs_per_lane 2 or 4 and LDS-staged links"), 1;
   const int B = c.tile.block;
   if (B < 64 || B > 256 || B % 64) return fail(err, "block must be 64, 128, 192 or 256"), 1;
-  if (c.tile.trav_hi > 1) return fail(err, "traversal bits 3 and above are not defined"), 1;
+  if (c.tile.trav_hi > 3 || c.tile.trav_hi == 2)
+    return fail(err, "traversal bits 3 and above are defined only as bit 3 with bit 2 (column order split in x)"), 1;
   if (c.tile.trav_hi) {
     // Column order of the packed layout: t_block x z_block blocks (traversal
     // 6) or sites (traversal 4) in (y, z).
@@ -378,8 +385,9 @@ Tile site_major_tile(const Tile &t0, const Geom &g, int k, int sl)
   if (!t0.trav_hi) return t;
   const int R = t.R > 0 ? t.R : 1, B = t.block > 0 ? t.block : 256;
   const int f = t0.hcube ? 2 : 1;  // column extents in sites
-  const long long col_lanes = 1LL * (g.L[0] / 2) * (f * t0.tblock) * (f * t0.zblock) * g.L[3] * (k / R);
-  if (sl == SL_PACKED && k % R == 0 && col_lanes % B == 0) {
+  const int nx = (t0.trav_hi & 2) ? 2 : 1;
+  const long long col_lanes = 1LL * (g.L[0] / 2 / nx) * (f * t0.tblock) * (f * t0.zblock) * g.L[3] * (k / R);
+  if (sl == SL_PACKED && k % R == 0 && (g.L[0] / 2) % nx == 0 && col_lanes % B == 0) {
     // The same columns in sites (traversal 4).
     t.tblock = f * t0.tblock;
     t.zblock = f * t0.zblock;

@@ -109,7 +109,8 @@ struct Tile {
   int hcube = 0;   // 1: hypercube work mapping (2x2x2x2 blocks, one per 8 P lanes; tblock / zblock
                    // then count blocks)
   int trav_hi = 0; // traversal bits 2 and above: 1 = column order (tblock / zblock = a column's extent
-                   // in y / z, in sites, or in blocks with hcube; XCD x walks columns 8 g + x along t)
+                   // in y / z, in sites, or in blocks with hcube; XCD x walks columns 8 g + x along t),
+                   // 3 = the same with x split in two halves
 };
 
 // The tile a site-major kernel runs for tile t (hcube cleared): a

@@ -42,10 +42,11 @@ __device__ __forceinline__ void hc_decode(uint32_t hb, const DslashArgs &a, uint
   if (a.col_order) {
     const uint32_t q = fdiv(hb, a.cdiv_slab);  // column * nT + T
     const uint32_t b = hb - q * a.col_slab;
-    const uint32_t col = fdiv(q, a.cdiv_nT);
-    T = q - col * a.col_nT;
-    const uint32_t r = fdiv(b, a.div_subLx);
-    xh = b - r * a.subLx;
+    const uint32_t colx = fdiv(q, a.cdiv_nT);
+    T = q - colx * a.col_nT;
+    const uint32_t col = fdiv(colx, a.cdiv_nx);
+    const uint32_t r = fdiv(b, a.cdiv_sx);
+    xh = (colx - col * a.col_nx) * a.col_sx + (b - r * a.col_sx);
     const uint32_t zin = fdiv(r, a.cdiv_cy);
     const uint32_t yin = r - zin * a.col_cy;
     const uint32_t cz = fdiv(col, a.cdiv_ncy);

@@ -113,9 +113,12 @@ struct DslashArgs {
   // T = 0 to nT - 1, for g = 0, 1, ... (col_remap); the launch has
   // 8 ceil(col_ncol / 8) col_grp workgroups, the ones past the last column
   // idle.
+  // With traversal bit 3 the columns also split x in two (col_sx = subLx / 2
+  // checkerboard x indices each, col_nx = 2; otherwise col_sx = subLx,
+  // col_nx = 1), the x part of the column index running fastest.
   int col_order;
-  uint32_t col_cy, col_cz, col_ncol, col_ncy, col_nT, col_slab, col_grp;
-  FastDiv cdiv_slab, cdiv_nT, cdiv_cy, cdiv_ncy, cdiv_grp;
+  uint32_t col_cy, col_cz, col_ncol, col_ncy, col_nT, col_slab, col_grp, col_sx, col_nx;
+  FastDiv cdiv_slab, cdiv_nT, cdiv_cy, cdiv_ncy, cdiv_grp, cdiv_sx, cdiv_nx;
   uint32_t ngrid;  // workgroups launched: nblocks, or the padded column-order grid
   // Pointer layouts only. Kept last: the packed kernels read only the fields
   // above, which then share a few scalar-cache lines of the argument block
@@ -309,10 +312,11 @@ __device__ __forceinline__ void decode_site(uint32_t sl, const DslashArgs &a, ui
   if (a.col_order) {
     const uint32_t q = fdiv(sl, a.cdiv_slab);  // column * Lt + t
     const uint32_t b = sl - q * a.col_slab;
-    const uint32_t col = fdiv(q, a.cdiv_nT);
-    t = q - col * a.col_nT;
-    const uint32_t r = fdiv(b, a.div_subLx);
-    xh = b - r * a.subLx;
+    const uint32_t colx = fdiv(q, a.cdiv_nT);
+    t = q - colx * a.col_nT;
+    const uint32_t col = fdiv(colx, a.cdiv_nx);
+    const uint32_t r = fdiv(b, a.cdiv_sx);
+    xh = (colx - col * a.col_nx) * a.col_sx + (b - r * a.col_sx);
     const uint32_t zin = fdiv(r, a.cdiv_cy);
     const uint32_t yin = r - zin * a.col_cy;
     const uint32_t cz = fdiv(col, a.cdiv_ncy);
@@ -1375,10 +1379,13 @@ __device__ __forceinline__ void dslash_body(const DslashArgs &a, uint32_t hw_blk
     {
       // Lane gid - xstride holds the same RHS slot of logical site sl - 1,
       // which is x index xh - 1 of the same row (x is the fastest coordinate
-      // of every traversal order).
+      // of every traversal order) unless xh starts a row: xh = 0, or in a
+      // column order that splits x, the first x index of the column.
       const int xstride = SITE_MINOR ? 1 : int(a.P);
       const int lw = int(threadIdx.x & 63u);
-      g.xown = (lw < xstride || g.xh == 0) ? 1 : 0;
+      const uint32_t xu = uint32_t(g.xh);
+      const bool row_start = a.col_order ? xu == fdiv(xu, a.cdiv_sx) * a.col_sx : xu == 0;
+      g.xown = (lw < xstride || row_start) ? 1 : 0;
       g.xsrc = g.xown ? lw : lw - xstride;
     }
     if constexpr (site_blocked(GL) && !LDS)

@@ -131,7 +131,7 @@ void load_tune_file(std::map<TuneKey, Tile> &m, const std::string &path_s)
     t.ystream = v[15] & 1;
     t.hcube = (v[15] >> 1) & 1;
     t.trav_hi = v[15] >> 2;
-    if (t.trav_hi > 1 || (t.trav_hi && t.ystream)) continue;  // not a defined traversal
+    if (t.trav_hi > 3 || t.trav_hi == 2 || (t.trav_hi && t.ystream)) continue;  // not a defined traversal
     m[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = t;
   }
   fclose(f);
@@ -657,7 +657,8 @@ int qcuMrhsSetTile(const qcuMrhsGeom *gc, int prec, int nrhs, const qcuMrhsTile 
   const Tile t = tile_from_c(*tile);
   if (t.R != 1 && t.R != 2 && t.R != 4) return api_fail("rhs_per_lane must be 1, 2 or 4");
   if (t.block < 64 || t.block > 256 || t.block % 64) return api_fail("block must be 64, 128, 192 or 256");
-  if (t.trav_hi > 1) return api_fail("traversal bits 3 and above are not defined");
+  if (t.trav_hi > 3 || t.trav_hi == 2)
+    return api_fail("traversal bits 3 and above are defined only as bit 3 with bit 2 (column order split in x)");
   if (t.trav_hi && t.ystream) return api_fail("column order (traversal bit 2) is traversal 4 or 6");
   set_tile(g, prec, nrhs, t);
   return 0;

@@ -145,6 +145,7 @@ def test_hcube_rejects_bad_tiles(lib, dev):
                    ((2, 128, 1, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
                    ((2, 128, 0, 3, 0, 0, 0, HC), "t_block must divide Lt / 2"),
                    ((2, 128, 1, 1, 0, 0, 0, HC | 8), "traversal bits 3 and above"),
+                   ((2, 128, 1, 1, 0, 0, 0, HC | 16 | COL), "traversal bits 3 and above"),
                    ((2, 128, 1, 1, 1, 0, 0, HC | COL | 1), "traversal 4 or 6"),
                    ((2, 128, 1, 4, 1, 0, 0, HC | COL), "column order: t_block"),
                    ((2, 128, 1, 1, 2, 0, 0, HC | COL), "column order: z_block")):
@@ -370,6 +371,8 @@ def col_tiles(L, k, Rs=(1, 2, 4), blocks=(64, 128, 256)):
                 for cz in [d for d in (1, 2, 3, 6) if bz % d == 0]:
                     if (L[0] // 2 * cy * cz) % (block // (8 * P)) == 0:
                         yield (R, block, 1, cy, cz, 0, 0, HC | COL)
+                    if (L[0] // 4 * cy * cz) % (block // (8 * P)) == 0 and L[0] % 4 == 0:
+                        yield (R, block, 1, cy, cz, 0, 0, HC | COL | 8)  # x split in two
 
 
 def test_column_order_every_tile_fp32(lib, dev):

@@ -18,14 +18,17 @@ import pytest
 def decode(hb, A):
     subLx, bLy, zb, tb, nzb = A["subLx"], A["bLy"], A["zblock"], A["tblock"], A["nzb"]
     if A.get("col"):
-        # column order: ((column * nT + T) * slab + (z_in * cy + y_in) * subLx + xh)
-        cy, cz = A["col"]
-        q, b = divmod(hb, subLx * cy * cz)
-        col, T = divmod(q, A["Lt"] // 2)
-        r, xh = divmod(b, subLx)
+        # column order: ((column * nT + T) * slab + (z_in * cy + y_in) * sx + x_in),
+        # column = (z column * ny + y column) * nx + x column (x split in nx)
+        cy, cz, nx = (tuple(A["col"]) + (1,))[:3]
+        sx = subLx // nx
+        q, b = divmod(hb, sx * cy * cz)
+        colx, T = divmod(q, A["Lt"] // 2)
+        col, cx_i = divmod(colx, nx)
+        r, xin = divmod(b, sx)
         zin, yin = divmod(r, cy)
         cz_i, cy_i = divmod(col, bLy // cy)
-        return xh, cy_i * cy + yin, cz_i * cz + zin, T
+        return cx_i * sx + xin, cy_i * cy + yin, cz_i * cz + zin, T
     if A["ystream"]:
         q, xh = divmod(hb, subLx)
         q2, zin = divmod(q, zb)
@@ -65,9 +68,9 @@ def canonical(idx, A, p):
 
 def col_remap(hw, A, B, P):
     """Logical workgroup of hardware workgroup hw in column order (None: idle)."""
-    cy, cz = A["col"]
-    ncol = (A["bLy"] // cy) * (A["bLz"] // cz)
-    grp = (A["Lt"] // 2) * (A["subLx"] * cy * cz // (B // (8 * P)))
+    cy, cz, nx = (tuple(A["col"]) + (1,))[:3]
+    ncol = (A["bLy"] // cy) * (A["bLz"] // cz) * nx
+    grp = (A["Lt"] // 2) * (A["subLx"] // nx * cy * cz // (B // (8 * P)))
     i, x = hw >> 3, hw & 7
     g = i // grp
     col = 8 * g + x
@@ -137,10 +140,10 @@ def check_launch(L, P, B, tb, zb, ystream, p, skip, col=None):
         # The padded hardware grid maps onto every logical workgroup once; XCD
         # hw % 8 works on columns congruent to it, one slab after another with T
         # ascending, and the blocks of a slab stay inside its column.
-        cy, cz = col
-        ncol = (Ly // 2 // cy) * (Lz // 2 // cz)
+        cy, cz, nx = (tuple(col) + (1,))[:3]
+        ncol = (Ly // 2 // cy) * (Lz // 2 // cz) * nx
         nb = nlanes // B
-        grp = (Lt // 2) * (subLx * cy * cz // (B // (8 * P)))
+        grp = (Lt // 2) * (subLx // nx * cy * cz // (B // (8 * P)))
         assert nlanes % B == 0 and nb == ncol * grp
         ngrid = (ncol + 7) // 8 * 8 * grp
         got = [col_remap(hw, A, B, P) for hw in range(ngrid)]
@@ -155,11 +158,13 @@ def check_launch(L, P, B, tb, zb, ystream, p, skip, col=None):
                 if a_ // grp == b_ // grp:
                     assert (b_ % grp) // slab_wg >= (a_ % grp) // slab_wg
         for blk in range(nb):
-            c = blk // grp
+            c, cxi = divmod(blk // grp, nx)
             cyi, czi = c % (Ly // 2 // cy), c // (Ly // 2 // cy)
+            sx = subLx // nx
             for gid in range(blk * B, blk * B + B):
-                Y, Z = lanes[gid][4] // 2, lanes[gid][5] // 2
+                X, Y, Z = lanes[gid][3], lanes[gid][4] // 2, lanes[gid][5] // 2
                 assert cyi * cy <= Y < (cyi + 1) * cy and czi * cz <= Z < (czi + 1) * cz
+                assert cxi * sx <= X < (cxi + 1) * sx
 
 
 @pytest.mark.parametrize("L", [(4, 4, 4, 4), (2, 2, 2, 2), (8, 4, 6, 2), (6, 2, 4, 8)])
@@ -182,11 +187,12 @@ def test_column_order_lane_map(L):
     for P, B in ((1, 64), (2, 128), (8, 128), (8, 256), (16, 128)):
         by, bz = L[1] // 2, L[2] // 2
         for cy, cz in itertools.product([d for d in (1, 2, 3) if by % d == 0], [d for d in (1, 2, 3) if bz % d == 0]):
-            if (L[0] // 2 * cy * cz) % (B // (8 * P)):
-                continue
-            for p, skip in ((0, (0, 0, 0, 0)), (1, (1, 0, 1, 1))):
-                check_launch(L, P, B, 1, 0, 0, p, skip, col=(cy, cz))
-                n += 1
+            for nx in (1, 2):
+                if (L[0] // 2) % nx or (L[0] // 2 // nx * cy * cz) % (B // (8 * P)):
+                    continue
+                for p, skip in ((0, (0, 0, 0, 0)), (1, (1, 0, 1, 1))):
+                    check_launch(L, P, B, 1, 0, 0, p, skip, col=(cy, cz, nx))
+                    n += 1
     assert n > 0
 
 
